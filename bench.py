@@ -1,0 +1,167 @@
+"""Benchmark of the BPNN energy+force path on one MI355X.
+
+Workload (default): 500 periodic boxes of 64 rigid H2O molecules (192 atoms, 12.42 A cube,
+96 000 atoms), species-resolved ACSF (2 x 10 G2 + 3 x 4 G4 = 32 features, Rc = 6.5 A), one
+32-25-25-1 tanh network per element (H, O), seeded random weights, z-score over the batch.
+
+One step = one full energy+force evaluation of the whole batch with the inputs resident in
+HBM: periodic neighbour list, ACSF, element networks with the input gradient, adjoint forces and
+the FP64 per-structure energy reduction (fnet_run).  ``value`` is atoms per second of the
+batched FP32 path; the plain path (FP64, one structure per launch) is timed on the same batch
+and reported beside it, as are the FP32-vs-FP64 energy and force errors.
+
+``roofline`` is for the dominant kernel (largest stage time measured with HIP events on the
+stream the kernels run on); ``cpu_baseline`` times the FP64 C oracle in the reference's loop
+shape (neighbour list rebuilt per atom and function, dense dG/dr, dense force contraction) on a
+few structures of the same workload, on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import fortnet_amd as fa  # noqa: E402
+from fortnet_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(stage, N, P, nG):
+    """Algorithmic HBM bytes of one launch over the batch (per-unit figures x units)."""
+    if stage == "neighbour":
+        return 24 * N + 4 * N + 8 * P + 4 * (N + 1)
+    if stage == "acsf":
+        return 12 * N + 4 * N + 8 * P + 4 * nG * N
+    if stage == "adjoint":
+        return 12 * N + 4 * N + 8 * P + 4 * nG * N + 12 * N
+    if stage == "networks":
+        return 4 * nG * N + 4 * N + 4 * nG * N
+    return 8 * N
+
+
+def cpu_baseline(geos, ns, budget_s=20.0):
+    from oracle import oracle as O
+    gf = O.GFunctions(ns.g.type, ns.g.rcut, ns.g.eta, ns.g.rs, ns.g.kappa, ns.g.lam, ns.g.xi,
+                      ns.g.atomid, ns.g.z1, ns.g.z2)
+    nets = O.Nets(ns.dims, ns.activation, ns.params)
+    atoms = 0
+    t0 = time.perf_counter()
+    done = 0
+    for g in geos:
+        st = O.Structure(g.coords, g.latvecs, int(g.periodic), g.atnum)
+        G = O.acsf(st, gf, O.REFSHAPE)
+        X = O.zscore(G, ns.zmean, ns.zstd)
+        nets.predict(X, g.species)
+        J = nets.jacobian(X, g.species)
+        dG = O.acsf_prime(st, gf, O.REFSHAPE) * O.zscale(ns.zstd, gf.n)[None, None, :, None]
+        F = -np.einsum("ia,kiac->kc", J[:, 0, :], dG)
+        atoms += g.n
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": atoms / dt, "unit": "atoms/s", "cores": 1, "kind": "port",
+            "sample": "%d of the workload's structures (%d atoms), FP64 C oracle in the "
+                      "reference loop shape (per-atom/per-function neighbour rebuild, dense "
+                      "dG/dr, dense force contraction), %.1f s" % (done, atoms, dt)}
+
+
+def time_steps(batch, steps, warmup):
+    for _ in range(warmup):
+        batch.run()
+    batch.sync()
+    stage = np.zeros(5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+        stage += batch.stage_times()
+    batch.sync()
+    dt = time.perf_counter() - t0
+    return dt, stage / max(steps, 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="water",
+                    choices=["water", "si64", "si_cells", "si_bulk", "ternary"])
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(structures shard with no exchange for inference; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    geos, ns, desc = W.config(args.workload, args.scale)
+    ctx = fa.Context(0)
+    ctx.upload_model(ns)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    ns = W.with_zscore(ns, b.acsf())
+    ctx.upload_model(ns)
+    N = b.N
+    nG = len(ns.g)
+
+    # accuracy of the batched FP32 path against the FP64 device path (itself pinned to the oracle)
+    b.set_mode(fa.FNET_FP64, fa.FNET_BATCHED)
+    E64, _ = b.energy()
+    F64 = b.forces()
+    b.set_mode(fa.FNET_FP32, fa.FNET_BATCHED)
+    E32, _ = b.energy()
+    F32 = b.forces()
+    nat = np.array([g.n for g in geos], float)
+    e_err = float((np.abs(E32[:, 0] - E64[:, 0]) / nat).max())
+    f_rmse = float(np.sqrt(np.mean((F32 - F64) ** 2)) / fa.BOHR__AA)
+
+    # batched FP32 path (the metric)
+    P = b.neighbour_list()
+    dt, stage = time_steps(b, args.steps, args.warmup)
+    value = args.steps * N / dt
+    ms_per_step = 1e3 * dt / args.steps
+
+    # plain path: FP64, one structure per launch
+    b.set_mode(fa.FNET_FP64, fa.FNET_PER_STRUCT)
+    dtp, stagep = time_steps(b, args.plain_steps, 1)
+    plain = args.plain_steps * N / dtp
+
+    names = ["neighbour", "acsf", "networks", "adjoint", "reduce"]
+    k = int(np.argmax(stage))
+    ab = algorithmic_bytes(names[k], N, P, nG)
+    achieved = ab / (stage[k] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic_bytes": ab, "kernel_ms": float(stage[k])}
+
+    out = {
+        "metric": "atoms/s energy+forces, batched vs plain GPU @1e-5 eV/atom; serve p50/p99 ms",
+        "value": value, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded structures, random-init weights)",
+        "config": {"workload": args.workload, "description": desc, "n_structures": len(geos),
+                   "atoms": N, "pairs": P, "n_features": nG,
+                   "precision": "fp32 ACSF/networks/forces, fp64 energy reduction"},
+        "plain_fp64_per_structure_atoms_per_s": plain,
+        "batched_over_plain": value / plain,
+        "fp32_vs_fp64": {"max_energy_err_ev_per_atom": e_err, "force_rmse_ev_per_A": f_rmse},
+        "stages_ms": dict(zip(names, [float(x) for x in stage])),
+        "plain_stages_ms": dict(zip(names, [float(x) for x in stagep])),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(geos, ns, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

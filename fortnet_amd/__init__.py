@@ -1,0 +1,9 @@
+"""MI355X (gfx950) implementation of Fortnet's BPNN hot path.
+
+ACSF descriptors over periodic neighbour lists, per-element networks, analytic forces and
+weight gradients run in libfnet_hip.so (hand-written HIP kernels); this package is the host
+side: model/batch management mirroring the reference's TAcsf/TBpnn/TNetwork data model.
+"""
+from ._lib import FNET_BATCHED, FNET_FP32, FNET_FP64, FNET_PER_STRUCT, FnetError  # noqa: F401
+from .core import (AA__BOHR, ACTIVATIONS, BOHR__AA, Batch, Context, GFunctions,  # noqa: F401
+                   Geometry, Netstat, nparams)

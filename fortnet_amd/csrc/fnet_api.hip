@@ -1,0 +1,876 @@
+// C-ABI of libfnet_hip.so: context, model and batch management and the stage drivers.
+// Declarations and their reference counterparts are in include/fnet_hip.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "../../include/fnet_hip.h"
+#include "fnet_internal.hpp"
+
+using namespace fnet;
+
+namespace {
+
+int fail(fnet_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hipfail(fnet_ctx* ctx, hipError_t e, const char* where) {
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  return fail(ctx, FNET_ERR_HIP, m);
+}
+
+#define HIPCHECK(ctx, expr)                                   \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) return hipfail((ctx), _e, #expr);   \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, sizeof(T) * count);
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+template <typename T>
+hipError_t upload(T** p, const T* h, size_t count, hipStream_t s) {
+  hipError_t e = dalloc(p, count);
+  if (e != hipSuccess) return e;
+  if (count && h) return hipMemcpyAsync(*p, h, sizeof(T) * count, hipMemcpyHostToDevice, s);
+  return hipSuccess;
+}
+
+void free_model(DevModel& m) {
+  dfree(m.radial);
+  dfree(m.groups);
+  dfree(m.zmean);
+  dfree(m.zinv);
+  dfree(m.ext_in);
+  dfree(m.params);
+  dfree(m.params_f);
+  m.radial = nullptr;
+  m.groups = nullptr;
+  m.zmean = m.zinv = nullptr;
+  m.ext_in = nullptr;
+  m.params = nullptr;
+  m.params_f = nullptr;
+  m.valid = false;
+}
+
+int zcode_of(const DevModel& m, int z) {
+  for (int c = 0; c < m.nZ; ++c)
+    if (m.zlist[c] == z) return c;
+  return -2;
+}
+
+void invalidate(fnet_batch* b) {
+  b->have_nl = b->have_G = b->have_E = b->have_F = false;
+}
+
+// (Re)builds the tiles of the network kernels for the batch's launch mode and precision.
+int ensure_tiles(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  const int TA = mlp_tile_atoms(m.shape, b->precision == FNET_FP64);
+  if (b->tile_atoms == TA && b->tile_launch == b->launch && b->tile_sp) return FNET_OK;
+  std::vector<int> perm, tsp, tstart, tcnt;
+  perm.reserve(b->N);
+  b->h_tile_first.assign(b->S + 1, 0);
+  auto add_tiles = [&](int sp, const std::vector<int>& atoms) {
+    for (size_t k = 0; k < atoms.size(); k += TA) {
+      tsp.push_back(sp);
+      tstart.push_back((int)perm.size());
+      const int c = (int)std::min<size_t>(TA, atoms.size() - k);
+      tcnt.push_back(c);
+      for (int q = 0; q < c; ++q) perm.push_back(atoms[k + q]);
+    }
+  };
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int s = 0; s < b->S; ++s) {
+      b->h_tile_first[s] = (int)tsp.size();
+      for (int sp = 0; sp < m.nspecies; ++sp) {
+        std::vector<int> atoms;
+        for (long long i = b->h_off[s]; i < b->h_off[s + 1]; ++i)
+          if (b->h_species[i] == sp) atoms.push_back((int)i);
+        add_tiles(sp, atoms);
+      }
+    }
+    b->h_tile_first[b->S] = (int)tsp.size();
+  } else {
+    std::vector<std::vector<int>> by(m.nspecies);
+    for (long long i = 0; i < b->N; ++i) by[b->h_species[i]].push_back((int)i);
+    for (int sp = 0; sp < m.nspecies; ++sp) add_tiles(sp, by[sp]);
+  }
+  dfree(b->perm);
+  dfree(b->tile_sp);
+  dfree(b->tile_start);
+  dfree(b->tile_cnt);
+  b->perm = b->tile_sp = b->tile_start = b->tile_cnt = nullptr;
+  hipStream_t s = ctx->stream;
+  HIPCHECK(ctx, upload(&b->perm, perm.data(), perm.size(), s));
+  HIPCHECK(ctx, upload(&b->tile_sp, tsp.data(), tsp.size(), s));
+  HIPCHECK(ctx, upload(&b->tile_start, tstart.data(), tstart.size(), s));
+  HIPCHECK(ctx, upload(&b->tile_cnt, tcnt.data(), tcnt.size(), s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  b->ntiles = (int)tsp.size();
+  b->tile_atoms = TA;
+  b->tile_launch = b->launch;
+  return FNET_OK;
+}
+
+int ensure_results(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  if (b->G) return FNET_OK;
+  HIPCHECK(ctx, dalloc((double**)&b->G, (size_t)b->N * m.nG));
+  HIPCHECK(ctx, dalloc((double**)&b->cG, (size_t)b->N * m.nout * m.nG));
+  HIPCHECK(ctx, dalloc(&b->Eatom, (size_t)b->N * m.nout));
+  HIPCHECK(ctx, dalloc(&b->Estruct, (size_t)b->S * m.nout));
+  HIPCHECK(ctx, dalloc((double**)&b->F, (size_t)b->N * m.nout * 3));
+  HIPCHECK(ctx, dalloc(&b->lossgrad, (size_t)b->N * m.nout));
+  return FNET_OK;
+}
+
+void ev_record(fnet_ctx* ctx, int k) {
+  if (k < 8) (void)hipEventRecord(ctx->ev[k], ctx->stream);
+}
+
+int do_neighbours(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  hipStream_t s = ctx->stream;
+  const double rc = m.rcmax;
+  HIPCHECK(ctx, launch_struct_prep(b, rc, s));
+  HIPCHECK(ctx, hipMemsetAsync(b->d_maxnb, 0, sizeof(int), s));
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st)
+      HIPCHECK(ctx, launch_neigh_count(b, rc, b->h_off[st], b->h_off[st + 1], s));
+  } else {
+    HIPCHECK(ctx, launch_neigh_count(b, rc, 0, b->N, s));
+  }
+  HIPCHECK(ctx, launch_scan(b->cnt, b->rowptr, b->N, b->scan_tmp, s));
+  int hv[2] = {0, 0};
+  HIPCHECK(ctx, hipMemcpyAsync(&hv[0], b->rowptr + b->N, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(ctx, hipMemcpyAsync(&hv[1], b->d_maxnb, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  const long long P = hv[0];
+  if (P < 0) return fail(ctx, FNET_ERR_CAPACITY, "neighbour list exceeds 2^31 entries");
+  b->max_nb = hv[1];
+  if (b->max_nb > 65535)
+    return fail(ctx, FNET_ERR_CAPACITY, "more than 65535 neighbours around one atom");
+  if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > 64 * 1024)
+    return fail(ctx, FNET_ERR_CAPACITY,
+                "too many neighbours around one atom for the LDS staging (" +
+                    std::to_string(b->max_nb) + ")");
+  if (P > b->pcap) {
+    dfree(b->nj);
+    dfree(b->nimg);
+    dfree(b->nd);
+    b->nj = b->nimg = nullptr;
+    b->nd = nullptr;
+    const long long cap = P + P / 8 + 1024;
+    HIPCHECK(ctx, dalloc(&b->nj, cap));
+    HIPCHECK(ctx, dalloc(&b->nimg, cap));
+    HIPCHECK(ctx, dalloc((double4**)&b->nd, cap));
+    b->pcap = cap;
+  }
+  b->npairs = P;
+  const int fp64 = b->precision == FNET_FP64;
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st)
+      HIPCHECK(ctx, launch_neigh_fill(b, rc, b->h_off[st], b->h_off[st + 1], fp64, s));
+  } else {
+    HIPCHECK(ctx, launch_neigh_fill(b, rc, 0, b->N, fp64, s));
+  }
+  b->have_nl = true;
+  b->result_precision = b->precision;
+  return FNET_OK;
+}
+
+int do_acsf(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  const int fp64 = b->precision == FNET_FP64;
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st)
+      HIPCHECK(ctx, launch_acsf_fwd(b, m, b->h_off[st], b->h_off[st + 1], fp64, ctx->stream));
+  } else {
+    HIPCHECK(ctx, launch_acsf_fwd(b, m, 0, b->N, fp64, ctx->stream));
+  }
+  b->have_G = true;
+  return FNET_OK;
+}
+
+int do_mlp(fnet_batch* b, int gradin) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  const int fp64 = b->precision == FNET_FP64;
+  int rc = ensure_tiles(b);
+  if (rc) return rc;
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st) {
+      const int t0 = b->h_tile_first[st], t1 = b->h_tile_first[st + 1];
+      HIPCHECK(ctx, launch_mlp_fwd(b, m, t0, t1 - t0, fp64, gradin, ctx->stream));
+    }
+  } else {
+    HIPCHECK(ctx, launch_mlp_fwd(b, m, 0, b->ntiles, fp64, gradin, ctx->stream));
+  }
+  return FNET_OK;
+}
+
+int do_reduce(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st)
+      HIPCHECK(ctx, launch_energy_reduce(b, m, st, st + 1, ctx->stream));
+  } else {
+    HIPCHECK(ctx, launch_energy_reduce(b, m, 0, b->S, ctx->stream));
+  }
+  b->have_E = true;
+  return FNET_OK;
+}
+
+int do_adjoint(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  const int fp64 = b->precision == FNET_FP64;
+  const size_t el = fp64 ? sizeof(double) : sizeof(float);
+  HIPCHECK(ctx, hipMemsetAsync(b->F, 0, el * (size_t)b->N * m.nout * 3, ctx->stream));
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st)
+      HIPCHECK(ctx, launch_acsf_adj(b, m, b->h_off[st], b->h_off[st + 1], fp64, ctx->stream));
+  } else {
+    HIPCHECK(ctx, launch_acsf_adj(b, m, 0, b->N, fp64, ctx->stream));
+  }
+  b->have_F = true;
+  return FNET_OK;
+}
+
+int check_ready(fnet_ctx* ctx, fnet_batch* b) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!b) return fail(ctx, FNET_ERR_ARG, "batch is NULL");
+  if (b->ctx != ctx) return fail(ctx, FNET_ERR_ARG, "batch belongs to another context");
+  if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "no model uploaded");
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  return ensure_results(b);
+}
+
+int download_T(fnet_ctx* ctx, const void* dsrc, int fp64, size_t n, double* out) {
+  if (fp64) {
+    HIPCHECK(ctx, hipMemcpyAsync(out, dsrc, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHECK(ctx, hipMemcpyAsync(tmp.data(), dsrc, sizeof(float) * n, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < n; ++k) out[k] = (double)tmp[k];
+  }
+  return FNET_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fnet_ctx_create(int device, fnet_ctx** out) {
+  if (!out) return FNET_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FNET_ERR_HIP;
+  if (device < 0 || device >= ndev) return FNET_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return FNET_ERR_HIP;
+  fnet_ctx* ctx = new fnet_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return FNET_ERR_HIP;
+  }
+  for (int k = 0; k < 8; ++k) (void)hipEventCreate(&ctx->ev[k]);
+  *out = ctx;
+  return FNET_OK;
+}
+
+void fnet_ctx_destroy(fnet_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_model(ctx->model);
+  dfree(ctx->d_grad);
+  for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ctx->ev[k]);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int fnet_last_error(const fnet_ctx* ctx, char* buf, int len) {
+  if (!ctx || !buf || len <= 0) return FNET_ERR_ARG;
+  std::snprintf(buf, (size_t)len, "%s", ctx->err.c_str());
+  return FNET_OK;
+}
+
+long long fnet_params_per_species(int n_layers, const int* dims) {
+  if (n_layers < 2 || !dims) return -1;
+  long long nw = 0, nb = 0;
+  for (int l = 0; l < n_layers - 1; ++l) nw += (long long)dims[l] * dims[l + 1];
+  nw += dims[n_layers - 1];
+  for (int l = 0; l < n_layers; ++l) nb += dims[l];
+  return nw + nb;
+}
+
+int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!md) return fail(ctx, FNET_ERR_ARG, "model descriptor is NULL");
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const int nG = md->n_g;
+  if (nG < 0 || (nG > 0 && (!md->g_type || !md->g_rcut)))
+    return fail(ctx, FNET_ERR_ARG, "invalid ACSF table");
+  if (md->n_species <= 0 || md->n_species > 64)
+    return fail(ctx, FNET_ERR_ARG, "n_species must be in 1..64");
+  if (md->n_layers < 2 || md->n_layers > kMaxLayers)
+    return fail(ctx, FNET_ERR_ARG, "n_layers must be in 2..8");
+  if (!md->dims || !md->params || !md->activation || !md->species_z)
+    return fail(ctx, FNET_ERR_ARG, "network arrays missing");
+  if (md->dims[0] != nG + md->n_ext_in)
+    return fail(ctx, FNET_ERR_ARG, "dims[0] must equal n_g + n_ext_in");
+  for (int l = 0; l < md->n_layers; ++l)
+    if (md->dims[l] <= 0 || md->dims[l] > kMaxDim)
+      return fail(ctx, FNET_ERR_ARG, "layer width must be in 1..256");
+  if (md->dims[md->n_layers - 1] > kMaxOut)
+    return fail(ctx, FNET_ERR_ARG, "at most 8 network outputs");
+  DevModel m;
+  m.nG = nG;
+  m.nspecies = md->n_species;
+  m.nout = md->dims[md->n_layers - 1];
+  m.nin = md->dims[0];
+  m.next_in = md->n_ext_in;
+  // element table
+  std::vector<int> zs;
+  for (int sp = 0; sp < md->n_species; ++sp) zs.push_back(md->species_z[sp]);
+  for (int g = 0; g < nG; ++g) {
+    if (md->g_z1 && md->g_z1[g]) zs.push_back(md->g_z1[g]);
+    if (md->g_z2 && md->g_z2[g]) zs.push_back(md->g_z2[g]);
+  }
+  std::sort(zs.begin(), zs.end());
+  zs.erase(std::unique(zs.begin(), zs.end()), zs.end());
+  if ((int)zs.size() > kMaxZ) return fail(ctx, FNET_ERR_CAPACITY, "more than 16 elements");
+  m.nZ = (int)zs.size();
+  for (int c = 0; c < m.nZ; ++c) m.zlist[c] = zs[c];
+  // functions
+  std::vector<RadialFn> rad;
+  std::vector<AngGroup> grp;
+  std::map<std::tuple<int, double, double, int, int, int>, int> open;
+  std::vector<double> zmean(nG, 0.0), zinv(nG, 1.0);
+  double rcmax = 0.0;
+  for (int g = 0; g < nG; ++g) {
+    const int ty = md->g_type[g];
+    const double rc = md->g_rcut[g];
+    if (!(rc > 0.0)) return fail(ctx, FNET_ERR_ARG, "cutoff must be positive");
+    rcmax = std::max(rcmax, rc);
+    const int z1 = md->g_z1 ? md->g_z1[g] : 0, z2 = md->g_z2 ? md->g_z2[g] : 0;
+    const int atomid = md->g_atomid ? md->g_atomid[g] : 0;
+    const double eta = md->g_eta ? md->g_eta[g] : 0.0;
+    if (atomid < 0) return fail(ctx, FNET_ERR_ARG, "negative atom id");
+    if (ty >= 1 && ty <= 3) {
+      RadialFn f{};
+      f.col = g;
+      f.type = ty;
+      f.zc = z1 ? zcode_of(m, z1) : -1;
+      f.atomid = atomid;
+      f.rc = rc;
+      f.eta = eta;
+      f.rs = md->g_rs ? md->g_rs[g] : 0.0;
+      f.kappa = md->g_kappa ? md->g_kappa[g] : 0.0;
+      rad.push_back(f);
+    } else if (ty == 4 || ty == 5) {
+      if ((z1 == 0) != (z2 == 0))
+        return fail(ctx, FNET_ERR_ARG, "angular function with one species of two unresolved");
+      const auto key = std::make_tuple(ty, rc, eta, z1, z2, atomid);
+      auto it = open.find(key);
+      if (it == open.end() || grp[it->second].nmem == kMaxMembers) {
+        AngGroup a{};
+        a.type = ty;
+        a.zc1 = z1 ? zcode_of(m, z1) : -1;
+        a.zc2 = z2 ? zcode_of(m, z2) : -1;
+        a.same = (z1 == z2) ? 1 : 0;
+        a.atomid = atomid;
+        a.rc = rc;
+        a.eta = eta;
+        grp.push_back(a);
+        open[key] = (int)grp.size() - 1;
+        it = open.find(key);
+      }
+      AngGroup& a = grp[it->second];
+      const double xi = md->g_xi ? md->g_xi[g] : 0.0;
+      const int q = a.nmem++;
+      a.col[q] = g;
+      a.xi[q] = xi;
+      a.lam[q] = md->g_lambda ? md->g_lambda[g] : 0.0;
+      a.xint[q] = (xi >= 0.0 && xi <= 64.0 && std::floor(xi) == xi) ? (int)xi : -1;
+      a.scale[q] = std::pow(2.0, 1.0 - xi);
+    } else {
+      return fail(ctx, FNET_ERR_ARG, "invalid G-function type");
+    }
+    if (md->z_mean && md->z_std && !(md->z_std[g] < 1e-8)) {
+      zmean[g] = md->z_mean[g];
+      zinv[g] = 1.0 / md->z_std[g];
+    }
+  }
+  m.nR = (int)rad.size();
+  m.nAG = (int)grp.size();
+  m.rcmax = rcmax;
+  // network shape
+  NetShape& sh = m.shape;
+  sh.nl = md->n_layers;
+  long long o = 0;
+  for (int l = 0; l < sh.nl; ++l) sh.dims[l] = md->dims[l];
+  for (int l = 0; l < sh.nl - 1; ++l) {
+    sh.woff[l] = o;
+    o += (long long)sh.dims[l] * sh.dims[l + 1];
+  }
+  sh.woff[sh.nl - 1] = o;
+  o += sh.dims[sh.nl - 1];
+  for (int l = 0; l < sh.nl; ++l) {
+    sh.boff[l] = o;
+    o += sh.dims[l];
+  }
+  sh.nparams = o;
+  for (int sp = 0; sp < md->n_species; ++sp) {
+    const int a = md->activation[sp];
+    if (a < 0 || a > 9) return fail(ctx, FNET_ERR_ARG, "invalid activation id");
+    sh.act[sp] = a;
+  }
+  std::vector<int> ext_in(md->n_ext_in);
+  for (int k = 0; k < md->n_ext_in; ++k) {
+    if (!md->ext_in_idx || md->ext_in_idx[k] < 1)
+      return fail(ctx, FNET_ERR_ARG, "external input indices are 1-based");
+    ext_in[k] = md->ext_in_idx[k] - 1;
+  }
+  const size_t np = (size_t)sh.nparams * md->n_species;
+  m.h_params.assign(md->params, md->params + np);
+  m.h_species_z.assign(md->species_z, md->species_z + md->n_species);
+  hipStream_t s = ctx->stream;
+  HIPCHECK(ctx, upload(&m.radial, rad.data(), rad.size(), s));
+  HIPCHECK(ctx, upload(&m.groups, grp.data(), grp.size(), s));
+  HIPCHECK(ctx, upload(&m.zmean, zmean.data(), zmean.size(), s));
+  HIPCHECK(ctx, upload(&m.zinv, zinv.data(), zinv.size(), s));
+  HIPCHECK(ctx, upload(&m.ext_in, ext_in.data(), ext_in.size(), s));
+  HIPCHECK(ctx, upload(&m.params, m.h_params.data(), np, s));
+  HIPCHECK(ctx, dalloc(&m.params_f, np));
+  HIPCHECK(ctx, launch_to_float(m.params, m.params_f, (long long)np, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  m.valid = true;
+  free_model(ctx->model);
+  ctx->model = m;
+  ctx->have_grad = false;
+  return FNET_OK;
+}
+
+int fnet_params_upload(fnet_ctx* ctx, const double* params) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "no model uploaded");
+  if (!params) return fail(ctx, FNET_ERR_ARG, "params is NULL");
+  DevModel& m = ctx->model;
+  const size_t np = (size_t)m.shape.nparams * m.nspecies;
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  m.h_params.assign(params, params + np);
+  HIPCHECK(ctx, hipMemcpyAsync(m.params, params, sizeof(double) * np, hipMemcpyHostToDevice,
+                               ctx->stream));
+  HIPCHECK(ctx, launch_to_float(m.params, m.params_f, (long long)np, ctx->stream));
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return FNET_OK;
+}
+
+int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* d, fnet_batch** out) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!d || !out) return fail(ctx, FNET_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "upload the model before a batch");
+  if (d->n_struct < 0 || !d->offsets) return fail(ctx, FNET_ERR_ARG, "invalid structure count");
+  const long long N = d->n_struct ? d->offsets[d->n_struct] : 0;
+  if (N < 0 || N >= (1LL << 30)) return fail(ctx, FNET_ERR_ARG, "invalid atom count");
+  if (d->offsets[0] != 0) return fail(ctx, FNET_ERR_ARG, "offsets[0] must be 0");
+  for (int s = 0; s < d->n_struct; ++s)
+    if (d->offsets[s + 1] < d->offsets[s]) return fail(ctx, FNET_ERR_ARG, "offsets decrease");
+  if (N > 0 && (!d->coords || !d->species || !d->atnum))
+    return fail(ctx, FNET_ERR_ARG, "coords/species/atnum missing");
+  if (d->n_ext < 0 || (d->n_ext > 0 && !d->ext)) return fail(ctx, FNET_ERR_ARG, "invalid ext");
+  const DevModel& m = ctx->model;
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  fnet_batch* b = new fnet_batch();
+  b->ctx = ctx;
+  b->S = d->n_struct;
+  b->N = N;
+  b->next = d->n_ext;
+  b->h_off.assign(d->offsets, d->offsets + d->n_struct + 1);
+  b->h_species.assign(d->species, d->species + N);
+  std::vector<int> per(d->n_struct, 0), astruct(N), zc(N);
+  std::vector<double> lat((size_t)9 * d->n_struct, 0.0);
+  for (int s = 0; s < d->n_struct; ++s) {
+    per[s] = d->periodic ? (d->periodic[s] != 0) : 0;
+    if (per[s]) {
+      if (!d->latvecs) {
+        delete b;
+        return fail(ctx, FNET_ERR_ARG, "periodic structure without lattice vectors");
+      }
+      for (int q = 0; q < 9; ++q) lat[9 * s + q] = d->latvecs[9 * s + q];
+      // fnetdata.F90:1151-1154 rejects dependent lattice vectors; the image ranges of
+      // latpointiter.F90:226-253 must also stay finite and modest
+      const double* L = &lat[9 * s];
+      auto o = [&](int r, int c) { return L[c * 3 + r]; };
+      const double det = o(0, 0) * (o(1, 1) * o(2, 2) - o(2, 1) * o(1, 2)) -
+                         o(0, 1) * (o(1, 0) * o(2, 2) - o(2, 0) * o(1, 2)) +
+                         o(0, 2) * (o(1, 0) * o(2, 1) - o(2, 0) * o(1, 1));
+      if (!(std::fabs(det) >= 1e-12)) {
+        delete b;
+        return fail(ctx, FNET_ERR_ARG, "dependent lattice vectors in structure " +
+                                           std::to_string(s));
+      }
+      double cof[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+          cof[r][c] = o(r1, c1) * o(r2, c2) - o(r1, c2) * o(r2, c1);
+        }
+      double nimg = 1.0;
+      for (int a = 0; a < 3; ++a) {
+        // row a of inv(L) is column a of the cofactor matrix divided by det
+        double ss = 0.0;
+        for (int c = 0; c < 3; ++c) ss += (cof[c][a] / det) * (cof[c][a] / det);
+        nimg *= 2.0 * (std::floor(m.rcmax * std::sqrt(ss)) + 1.0) + 1.0;
+      }
+      if (!(nimg <= 1.0e6)) {
+        delete b;
+        return fail(ctx, FNET_ERR_CAPACITY, "cell too small for the cutoff (more than 1e6 "
+                                            "lattice images) in structure " + std::to_string(s));
+      }
+    }
+    for (long long i = d->offsets[s]; i < d->offsets[s + 1]; ++i) astruct[i] = s;
+  }
+  for (long long i = 0; i < N; ++i) {
+    if (d->species[i] < 0 || d->species[i] >= m.nspecies) {
+      delete b;
+      return fail(ctx, FNET_ERR_ARG, "species index out of range");
+    }
+    zc[i] = zcode_of(m, d->atnum[i]);
+  }
+  // every ext column used by the model must exist
+  int need_ext = 0;
+  std::vector<int> ext_in(m.next_in);
+  if (m.next_in > 0)
+    (void)hipMemcpy(ext_in.data(), m.ext_in, sizeof(int) * m.next_in, hipMemcpyDeviceToHost);
+  for (int k = 0; k < m.next_in; ++k) need_ext = std::max(need_ext, ext_in[k] + 1);
+  {
+    std::vector<RadialFn> rad(m.nR);
+    std::vector<AngGroup> grp(m.nAG);
+    if (m.nR) (void)hipMemcpy(rad.data(), m.radial, sizeof(RadialFn) * m.nR, hipMemcpyDeviceToHost);
+    if (m.nAG) (void)hipMemcpy(grp.data(), m.groups, sizeof(AngGroup) * m.nAG, hipMemcpyDeviceToHost);
+    for (auto& f : rad) need_ext = std::max(need_ext, f.atomid);
+    for (auto& g : grp) need_ext = std::max(need_ext, g.atomid);
+  }
+  if (need_ext > d->n_ext) {
+    delete b;
+    return fail(ctx, FNET_ERR_ARG, "the model needs more external feature columns than given");
+  }
+  hipStream_t s = ctx->stream;
+  HIPCHECK(ctx, upload(&b->xyz, d->coords, (size_t)3 * N, s));
+  HIPCHECK(ctx, upload(&b->lat, lat.data(), lat.size(), s));
+  HIPCHECK(ctx, upload(&b->periodic, per.data(), per.size(), s));
+  HIPCHECK(ctx, upload(&b->off, d->offsets, (size_t)d->n_struct + 1, s));
+  HIPCHECK(ctx, upload(&b->atom_struct, astruct.data(), astruct.size(), s));
+  HIPCHECK(ctx, upload(&b->species, d->species, (size_t)N, s));
+  HIPCHECK(ctx, upload(&b->zcode, zc.data(), zc.size(), s));
+  HIPCHECK(ctx, upload(&b->ext, d->ext, (size_t)N * d->n_ext, s));
+  HIPCHECK(ctx, dalloc(&b->sinfo, (size_t)d->n_struct));
+  HIPCHECK(ctx, dalloc(&b->cnt, (size_t)N));
+  HIPCHECK(ctx, dalloc(&b->rowptr, (size_t)N + 1));
+  HIPCHECK(ctx, dalloc(&b->scan_tmp, (size_t)scan_tmp_size(N)));
+  HIPCHECK(ctx, dalloc(&b->d_maxnb, 1));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  *out = b;
+  return FNET_OK;
+}
+
+void fnet_batch_destroy(fnet_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  void* ptrs[] = {b->xyz, b->lat, b->periodic, b->off, b->atom_struct, b->species, b->zcode,
+                  b->ext, b->sinfo, b->perm, b->tile_sp, b->tile_start, b->tile_cnt, b->cnt,
+                  b->rowptr, b->scan_tmp, b->nj, b->nimg, b->nd, b->d_maxnb, b->G, b->cG,
+                  b->Eatom, b->Estruct, b->F, b->lossgrad, b->d_targets};
+  for (void* p : ptrs) dfree(p);
+  if (b->ctx->last_train_batch == b) b->ctx->last_train_batch = nullptr;
+  delete b;
+}
+
+int fnet_batch_update_coords(fnet_batch* b, const double* coords, const double* latvecs) {
+  if (!b) return FNET_ERR_ARG;
+  fnet_ctx* ctx = b->ctx;
+  if (!coords) return fail(ctx, FNET_ERR_ARG, "coords is NULL");
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  HIPCHECK(ctx, hipMemcpyAsync(b->xyz, coords, sizeof(double) * 3 * b->N, hipMemcpyHostToDevice,
+                               ctx->stream));
+  if (latvecs)
+    HIPCHECK(ctx, hipMemcpyAsync(b->lat, latvecs, sizeof(double) * 9 * b->S,
+                                 hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  invalidate(b);
+  return FNET_OK;
+}
+
+int fnet_batch_set_mode(fnet_batch* b, int precision, int launch) {
+  if (!b) return FNET_ERR_ARG;
+  if (precision != FNET_FP64 && precision != FNET_FP32)
+    return fail(b->ctx, FNET_ERR_ARG, "unknown precision");
+  if (launch != FNET_BATCHED && launch != FNET_PER_STRUCT)
+    return fail(b->ctx, FNET_ERR_ARG, "unknown launch mode");
+  if (b->precision != precision || b->launch != launch) invalidate(b);
+  b->precision = precision;
+  b->launch = launch;
+  return FNET_OK;
+}
+
+long long fnet_batch_n_atoms(const fnet_batch* b) { return b ? b->N : -1; }
+
+int fnet_neighbour_list(fnet_ctx* ctx, fnet_batch* b, long long* n_pairs) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  invalidate(b);
+  rc = do_neighbours(b);
+  if (rc) return rc;
+  if (n_pairs) *n_pairs = b->npairs;
+  return FNET_OK;
+}
+
+int fnet_get_neighbours(fnet_ctx* ctx, fnet_batch* b, long long* rowptr, int* j, int* img) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  if (!b->have_nl) return fail(ctx, FNET_ERR_STATE, "no neighbour list computed");
+  std::vector<int> rp(b->N + 1);
+  HIPCHECK(ctx, hipMemcpyAsync(rp.data(), b->rowptr, sizeof(int) * (b->N + 1),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  if (j && b->npairs)
+    HIPCHECK(ctx, hipMemcpyAsync(j, b->nj, sizeof(int) * b->npairs, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  if (img && b->npairs)
+    HIPCHECK(ctx, hipMemcpyAsync(img, b->nimg, sizeof(int) * b->npairs, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (rowptr)
+    for (long long i = 0; i <= b->N; ++i) rowptr[i] = rp[i];
+  return FNET_OK;
+}
+
+int fnet_acsf_forward(fnet_ctx* ctx, fnet_batch* b, double* G_out) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  if (!b->have_nl || b->result_precision != b->precision) {
+    rc = do_neighbours(b);
+    if (rc) return rc;
+  }
+  rc = do_acsf(b);
+  if (rc) return rc;
+  if (G_out)
+    return download_T(ctx, b->G, b->precision == FNET_FP64, (size_t)b->N * ctx->model.nG, G_out);
+  return FNET_OK;
+}
+
+int fnet_energy(fnet_ctx* ctx, fnet_batch* b, double* E_struct, double* E_atom) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  if (!b->have_G || b->result_precision != b->precision) {
+    rc = fnet_acsf_forward(ctx, b, nullptr);
+    if (rc) return rc;
+  }
+  if (!b->have_E) {
+    rc = do_mlp(b, 0);
+    if (rc) return rc;
+    rc = do_reduce(b);
+    if (rc) return rc;
+  }
+  const DevModel& m = ctx->model;
+  if (E_struct)
+    HIPCHECK(ctx, hipMemcpyAsync(E_struct, b->Estruct, sizeof(double) * b->S * m.nout,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  if (E_atom)
+    HIPCHECK(ctx, hipMemcpyAsync(E_atom, b->Eatom, sizeof(double) * b->N * m.nout,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return FNET_OK;
+}
+
+int fnet_forces(fnet_ctx* ctx, fnet_batch* b, double* F) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  if (!b->have_G || b->result_precision != b->precision) {
+    rc = fnet_acsf_forward(ctx, b, nullptr);
+    if (rc) return rc;
+  }
+  if (!b->have_F) {
+    rc = do_mlp(b, 1);
+    if (rc) return rc;
+    rc = do_reduce(b);
+    if (rc) return rc;
+    rc = do_adjoint(b);
+    if (rc) return rc;
+  }
+  if (F)
+    return download_T(ctx, b->F, b->precision == FNET_FP64, (size_t)b->N * ctx->model.nout * 3, F);
+  return FNET_OK;
+}
+
+int fnet_run(fnet_ctx* ctx, fnet_batch* b) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  invalidate(b);
+  rc = ensure_tiles(b);
+  if (rc) return rc;
+  ev_record(ctx, 0);
+  rc = do_neighbours(b);
+  if (rc) return rc;
+  ev_record(ctx, 1);
+  rc = do_acsf(b);
+  if (rc) return rc;
+  ev_record(ctx, 2);
+  rc = do_mlp(b, 1);
+  if (rc) return rc;
+  ev_record(ctx, 3);
+  rc = do_adjoint(b);
+  if (rc) return rc;
+  ev_record(ctx, 4);
+  rc = do_reduce(b);
+  if (rc) return rc;
+  ev_record(ctx, 5);
+  ctx->n_ev = 6;
+  return FNET_OK;
+}
+
+int fnet_stage_times(fnet_ctx* ctx, double* ms, int n) {
+  if (!ctx || !ms) return FNET_ERR_ARG;
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  int k = 0;
+  for (; k < n && k + 1 < ctx->n_ev; ++k) {
+    float t = 0.f;
+    HIPCHECK(ctx, hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]));
+    ms[k] = t;
+  }
+  return FNET_OK;
+}
+
+int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const double* g_targets,
+                         const double* a_targets, const double* atomic_weights,
+                         const double* dp_weights, double* grad, double* pred) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  const DevModel& m = ctx->model;
+  if (n_global < 0 || n_global > m.nout) return fail(ctx, FNET_ERR_ARG, "invalid n_global");
+  if (n_global > 0 && !g_targets) return fail(ctx, FNET_ERR_ARG, "g_targets missing");
+  if (n_global < m.nout && !a_targets) return fail(ctx, FNET_ERR_ARG, "a_targets missing");
+  if (!b->have_G || b->result_precision != b->precision) {
+    rc = fnet_acsf_forward(ctx, b, nullptr);
+    if (rc) return rc;
+  }
+  rc = do_mlp(b, 0);
+  if (rc) return rc;
+  rc = do_reduce(b);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  // stage inputs: gt [S*ng], at [N*na], aw [N], dpw [S]
+  const long long na = m.nout - n_global;
+  const long long need = (long long)b->S * n_global + b->N * na + b->N + b->S + 4;
+  if (need > b->targets_cap) {
+    dfree(b->d_targets);
+    b->d_targets = nullptr;
+    HIPCHECK(ctx, dalloc(&b->d_targets, (size_t)need));
+    b->targets_cap = need;
+  }
+  double* gt = b->d_targets;
+  double* at = gt + (long long)b->S * n_global;
+  double* aw = at + b->N * na;
+  double* dpw = aw + b->N;
+  double wsum = 0.0;
+  if (n_global > 0)
+    HIPCHECK(ctx, hipMemcpyAsync(gt, g_targets, sizeof(double) * b->S * n_global,
+                                 hipMemcpyHostToDevice, s));
+  if (na > 0)
+    HIPCHECK(ctx, hipMemcpyAsync(at, a_targets, sizeof(double) * b->N * na, hipMemcpyHostToDevice,
+                                 s));
+  if (atomic_weights)
+    HIPCHECK(ctx, hipMemcpyAsync(aw, atomic_weights, sizeof(double) * b->N,
+                                 hipMemcpyHostToDevice, s));
+  if (dp_weights) {
+    HIPCHECK(ctx, hipMemcpyAsync(dpw, dp_weights, sizeof(double) * b->S, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < b->S; ++k) wsum += dp_weights[k];
+  } else {
+    wsum = (double)b->S;
+  }
+  if (!(wsum > 0.0)) return fail(ctx, FNET_ERR_ARG, "datapoint weights sum to zero");
+  HIPCHECK(ctx, launch_lossgrad(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
+                                dp_weights ? dpw : nullptr, 1.0 / wsum, s));
+  const long long np = m.shape.nparams * m.nspecies;
+  if (np > ctx->grad_cap) {
+    dfree(ctx->d_grad);
+    ctx->d_grad = nullptr;
+    HIPCHECK(ctx, dalloc(&ctx->d_grad, (size_t)np + m.nspecies));
+    ctx->grad_cap = np;
+  }
+  HIPCHECK(ctx, hipMemsetAsync(ctx->d_grad, 0, sizeof(double) * np, s));
+  HIPCHECK(ctx, launch_mlp_grad(b, m, b->precision == FNET_FP64, ctx->d_grad, s));
+  if (grad)
+    HIPCHECK(ctx, hipMemcpyAsync(grad, ctx->d_grad, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+  if (pred)
+    HIPCHECK(ctx, hipMemcpyAsync(pred, b->Eatom, sizeof(double) * b->N * m.nout,
+                                 hipMemcpyDeviceToHost, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  ctx->have_grad = true;
+  ctx->last_train_batch = b;
+  return FNET_OK;
+}
+
+int fnet_optimizer_step_sd(fnet_ctx* ctx, double lr, double max_disp, double tol,
+                           double* params_out, int* converged) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "no model uploaded");
+  if (!ctx->have_grad) return fail(ctx, FNET_ERR_STATE, "no gradient computed");
+  if (!(max_disp > 0.0)) return fail(ctx, FNET_ERR_ARG, "max_disp must be positive");
+  DevModel& m = ctx->model;
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  const long long np = m.shape.nparams * m.nspecies;
+  double* gmax = ctx->d_grad + np;
+  HIPCHECK(ctx, launch_sd_step(m.params, ctx->d_grad, m.shape.nparams, lr, max_disp, tol, gmax,
+                               m.nspecies, ctx->stream));
+  HIPCHECK(ctx, launch_to_float(m.params, m.params_f, np, ctx->stream));
+  std::vector<double> gm(m.nspecies);
+  HIPCHECK(ctx, hipMemcpyAsync(m.h_params.data(), m.params, sizeof(double) * np,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(ctx, hipMemcpyAsync(gm.data(), gmax, sizeof(double) * m.nspecies,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (params_out) std::memcpy(params_out, m.h_params.data(), sizeof(double) * np);
+  if (converged) {
+    int c = 1;
+    for (double v : gm) c = c && (v < tol);
+    *converged = c;
+  }
+  ctx->have_grad = false;
+  return FNET_OK;
+}
+
+int fnet_sync(fnet_ctx* ctx) {
+  if (!ctx) return FNET_ERR_ARG;
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return FNET_OK;
+}
+
+}  // extern "C"

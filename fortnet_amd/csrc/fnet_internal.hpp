@@ -1,0 +1,198 @@
+// Internal declarations shared by the HIP translation units of libfnet_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+namespace fnet {
+
+constexpr int kMaxLayers = 8;   // input + hidden + output layers of one element network
+constexpr int kMaxDim = 256;    // widest layer of the generic network kernels
+constexpr int kMaxMembers = 16; // functions sharing one angular neighbour-pair loop
+constexpr int kMaxZ = 16;       // distinct atomic numbers referenced by one model
+constexpr int kMaxOut = 8;      // network outputs (targets)
+constexpr double kPi = 3.14159265358979323846; // lib_dftbp/constants.F90:15
+
+// One radial function (G1/G2/G3), acsf.F90:1287-1373.
+struct RadialFn {
+  int col;     // output column in G
+  int type;    // 1, 2, 3
+  int zc;      // element code of neighbour species, -1 = any
+  int atomid;  // 0 = none, else 1-based ext column
+  double rc, eta, rs, kappa;
+};
+
+// Angular functions (G4/G5, acsf.F90:1377-1492) that share type, cutoff, eta, species pair and
+// atom-id column: they are evaluated in one pass over the neighbour pairs.
+struct AngGroup {
+  int type;   // 4 or 5
+  int zc1, zc2;
+  int same;   // both neighbour lists identical (unresolved or Z1 == Z2): ordered pairs + diagonal
+  int atomid;
+  int nmem;
+  double rc, eta;
+  int col[kMaxMembers];
+  int xint[kMaxMembers]; // xi as an integer power, or -1 for a general real power
+  double xi[kMaxMembers];
+  double lam[kMaxMembers];
+  double scale[kMaxMembers]; // 2^(1 - xi)
+};
+
+// Network shape of every element network.
+struct NetShape {
+  int nl;                  // number of layers
+  int dims[kMaxLayers];
+  long long woff[kMaxLayers]; // offset of ww(l) (d_l x d_{l+1}) in the serialised vector
+  long long boff[kMaxLayers]; // offset of bb(l)
+  long long nparams;
+  int act[64];             // hidden activation per species (<= 64 species)
+};
+
+// Per-structure neighbour-search data, computed on the device each step.
+struct StructInfo {
+  double inv[9];    // inverse lattice, inv[c*3+r] = inv(r,c)
+  double smin[3], smax[3]; // extent of fractional coordinates
+  double w[3];      // rc * |row_a(inv L)|: largest fractional offset of a neighbour
+  int lo[3], hi[3]; // reference image ranges (latpointiter.F90:226-253)
+  int periodic;
+  int pad;
+};
+
+// Device copy of the model.
+struct DevModel {
+  int nG = 0, nR = 0, nAG = 0, nZ = 0;
+  int zlist[kMaxZ];
+  int nspecies = 0, nout = 0, nin = 0, next_in = 0;
+  double rcmax = 0.0;
+  NetShape shape{};
+  RadialFn* radial = nullptr;   // [nR]
+  AngGroup* groups = nullptr;   // [nAG]
+  double* zmean = nullptr;      // [nG] (0 where no standardisation)
+  double* zinv = nullptr;       // [nG] (1 where no standardisation)
+  int* ext_in = nullptr;        // [next_in] 0-based ext columns
+  double* params = nullptr;     // [nspecies][nparams]
+  float* params_f = nullptr;    // FP32 copy
+  std::vector<double> h_params;
+  std::vector<int> h_species_z;
+  bool valid = false;
+};
+
+struct Timer {
+  hipEvent_t ev[8];
+  bool made = false;
+};
+
+}  // namespace fnet
+
+struct fnet_batch;
+
+struct fnet_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  fnet::DevModel model;
+  hipEvent_t ev[8] = {};
+  int n_ev = 0;
+  double stage_ms[8] = {};
+  double* d_grad = nullptr; // [nspecies][nparams] FP64 gradient of the last train call
+  long long grad_cap = 0;
+  bool have_grad = false;
+  fnet_batch* last_train_batch = nullptr;
+};
+
+struct fnet_batch {
+  fnet_ctx* ctx = nullptr;
+  int S = 0;
+  long long N = 0;
+  int next = 0;
+  int precision = 1;
+  int launch = 0;
+  std::vector<long long> h_off;
+  std::vector<int> h_species;
+  // inputs
+  double* xyz = nullptr;       // [N][3]
+  double* lat = nullptr;       // [S][9]
+  int* periodic = nullptr;     // [S]
+  long long* off = nullptr;    // [S+1]
+  int* atom_struct = nullptr;  // [N]
+  int* species = nullptr;      // [N]
+  int* zcode = nullptr;        // [N]
+  double* ext = nullptr;       // [N][next]
+  fnet::StructInfo* sinfo = nullptr; // [S]
+  // per-species tiles of the network kernels
+  int* perm = nullptr;         // [N] atoms grouped by species
+  int* tile_sp = nullptr;      // [ntiles]
+  int* tile_start = nullptr;   // [ntiles] into perm
+  int* tile_cnt = nullptr;     // [ntiles]
+  int ntiles = 0;
+  int tile_atoms = 0;
+  int tile_launch = -1;          // launch mode the tiles were built for
+  std::vector<int> h_tile_first; // [S+1] tiles of each structure (per-structure launches)
+  // neighbour list (CSR)
+  int* cnt = nullptr;          // [N]
+  int* rowptr = nullptr;       // [N+1]
+  int* scan_tmp = nullptr;
+  long long scan_tmp_cap = 0;
+  int* nj = nullptr;           // [P]
+  int* nimg = nullptr;         // [P]
+  void* nd = nullptr;          // [P] double4 or float4: image position - centre, and r
+  long long pcap = 0;
+  long long npairs = 0;
+  int max_nb = 0;
+  int* d_maxnb = nullptr;
+  // results
+  void* G = nullptr;           // [N][nG] T
+  void* cG = nullptr;          // [N][nout][nG] T
+  double* Eatom = nullptr;     // [N][nout]
+  double* Estruct = nullptr;   // [S][nout]
+  void* F = nullptr;           // [N][nout][3] T
+  double* lossgrad = nullptr;  // [N][nout]
+  double* d_targets = nullptr; // staging of training inputs
+  long long targets_cap = 0;
+  bool have_nl = false, have_G = false, have_E = false, have_F = false;
+  int result_precision = -1;
+};
+
+namespace fnet {
+
+// launchers (fnet_neigh.hip)
+hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s);
+hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
+hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
+                             hipStream_t s);
+hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s);
+long long scan_tmp_size(long long n);
+
+// launchers (fnet_acsf.hip)
+hipError_t launch_acsf_fwd(fnet_batch* b, const DevModel& m, long long a0, long long a1, int fp64,
+                           hipStream_t s);
+hipError_t launch_acsf_adj(fnet_batch* b, const DevModel& m, long long a0, long long a1, int fp64,
+                           hipStream_t s);
+
+// launchers (fnet_mlp.hip)
+hipError_t launch_mlp_fwd(fnet_batch* b, const DevModel& m, int first_tile, int ntiles, int fp64,
+                          int want_grad_in, hipStream_t s);
+hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1, hipStream_t s);
+hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
+                           const double* at, const double* aw, const double* dpw, double inv_wsum,
+                           hipStream_t s);
+hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad, hipStream_t s);
+hipError_t launch_sd_step(double* params, const double* grad, long long n, double lr,
+                          double max_disp, double tol, double* scratch, int nspecies,
+                          hipStream_t s);
+int mlp_tile_atoms(const NetShape& sh, int fp64);
+hipError_t launch_to_float(const double* in, float* out, long long n, hipStream_t s);
+
+size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj);
+
+}  // namespace fnet
+
+#define FNET_HIP_TRY(expr)                                   \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return _e;                         \
+  } while (0)

@@ -1,0 +1,454 @@
+// Element networks on gfx950: all atoms of one element form tiles; one block runs a tile through
+// every layer with the activations in LDS (network.F90:146-180 fprop, :314-347 iPredict), then
+// back to the input for the force adjoint (fdevi, network.F90:183-244, taken in reverse mode:
+// nOut reverse sweeps give the same Jacobian rows), or back through the weights for the
+// training gradient (bprop, network.F90:248-296, with the sysTrain normalisation of
+// bpnn.F90:692-702).  Per-structure energies and weight gradients are reduced in FP64.
+#include "fnet_internal.hpp"
+
+namespace fnet {
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T act_f(int id, T x) {
+  switch (id) {
+    case 7: return exp(-(x * x));                                   // gaussian
+    case 4: return log((T)1 + exp(x));                              // softplus
+    case 5: return (sqrt(x * x + (T)1) - (T)1) / (T)2 + x;          // bent identity
+    case 2: return x > (T)0 ? x : (T)0;                             // relu
+    case 3: return ((T)0.01 * x > x) ? (T)0.01 * x : x;             // leaky relu
+    case 1: return (T)1 / ((T)1 + exp(-x));                         // sigmoid
+    case 8: return x > (T)0 ? (T)1 : (T)0;                          // heaviside
+    case 6: return atan(x);                                         // atan
+    case 9: return x;                                               // linear
+    default: return tanh(x);                                        // tanh
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T act_d(int id, T x) {
+  switch (id) {
+    case 7: return (T)-2 * x * exp(-(x * x));
+    case 4: return (T)1 / ((T)1 + exp(-x));
+    case 5: return x / ((T)2 * sqrt(x * x + (T)1)) + (T)1;
+    case 2: return x >= (T)0 ? (T)1 : (T)0;
+    case 3: return x >= (T)0 ? (T)1 : (T)0.01;
+    case 1: {
+      const T s = (T)1 / ((T)1 + exp(-x));
+      return s * ((T)1 - s);
+    }
+    case 8: return (T)0;
+    case 6: return (T)1 / (x * x + (T)1);
+    case 9: return (T)1;
+    default: {
+      const T t = tanh(x);
+      return (T)1 - t * t;
+    }
+  }
+}
+
+struct MlpArgs {
+  NetShape sh;
+  const void* params; // [nspecies][nparams] T
+  const void* G;      // [N][nG] T
+  const double* ext;
+  int next;
+  const int* ext_in;  // [next_in]
+  int nG, next_in, nout;
+  const double* zmean;
+  const double* zinv;
+  const int* perm;
+  const int* tile_sp;
+  const int* tile_start;
+  const int* tile_cnt;
+  int first_tile;
+  int TA;
+  double* Eatom;      // [N][nout]
+  void* cG;           // [N][nout][nG] T
+  const double* lossgrad; // [N][nout]
+  double* grad;       // [nspecies][nparams]
+};
+
+// LDS per tile: input x [TA][d0], activations of every layer [TA][d_l], f'(z) of hidden layers,
+// and two delta buffers [TA][dmax].
+template <typename T>
+__device__ __forceinline__ void load_input(const MlpArgs& a, T* x, int cnt, int start) {
+  const int d0 = a.sh.dims[0];
+  for (int idx = threadIdx.x; idx < a.TA * d0; idx += blockDim.x) {
+    const int at = idx % a.TA, f = idx / a.TA;
+    T v = 0;
+    if (at < cnt) {
+      const long long atom = a.perm[start + at];
+      if (f < a.nG) {
+        v = (T)(((double)((const T*)a.G)[atom * a.nG + f] - a.zmean[f]) * a.zinv[f]);
+      } else {
+        v = (T)a.ext[atom * a.next + a.ext_in[f - a.nG]];
+      }
+    }
+    x[at * d0 + f] = v;
+  }
+}
+
+// fprop of one tile.  acts[l] = [TA][d_l] (acts[0] = x), fp[l] = f'(z_l) for hidden layers.
+template <typename T>
+__device__ void forward(const MlpArgs& a, const T* P, int act, T** acts, T** fp) {
+  const int nl = a.sh.nl;
+  for (int l = 1; l < nl; ++l) {
+    const int din = a.sh.dims[l - 1], dout = a.sh.dims[l];
+    const T* W = P + a.sh.woff[l - 1];
+    const T* bb = P + a.sh.boff[l];
+    const T* in = acts[l - 1];
+    T* out = acts[l];
+    const bool hidden = l < nl - 1;
+    for (int idx = threadIdx.x; idx < a.TA * dout; idx += blockDim.x) {
+      const int at = idx % a.TA, o = idx / a.TA;
+      const T* w = W + (long long)o * din;
+      const T* xi = in + at * din;
+      T z = 0;
+      for (int k = 0; k < din; ++k) z += w[k] * xi[k];
+      z += bb[o];
+      if (hidden) {
+        out[at * dout + o] = act_f(act, z);
+        fp[l][at * dout + o] = act_d(act, z);
+      } else {
+        out[at * dout + o] = z;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ size_t mlp_lds_floats(const NetShape& sh, int TA) {
+  size_t s = 0;
+  int dmax = 0;
+  for (int l = 0; l < sh.nl; ++l) {
+    s += sh.dims[l];
+    if (l > 0 && l < sh.nl - 1) s += sh.dims[l];
+    if (sh.dims[l] > dmax) dmax = sh.dims[l];
+  }
+  s += 2 * dmax;
+  return s * TA;
+}
+
+template <typename T, bool GRADIN>
+__global__ void __launch_bounds__(256) k_mlp_fwd(MlpArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = a.first_tile + blockIdx.x;
+  const int sp = a.tile_sp[tile], start = a.tile_start[tile], cnt = a.tile_cnt[tile];
+  const int nl = a.sh.nl;
+  const T* P = (const T*)a.params + (long long)sp * a.sh.nparams;
+  const int act = a.sh.act[sp];
+  T* acts[kMaxLayers];
+  T* fp[kMaxLayers];
+  T* p = (T*)smem;
+  int dmax = 0;
+  for (int l = 0; l < nl; ++l) {
+    acts[l] = p;
+    p += a.TA * a.sh.dims[l];
+    if (a.sh.dims[l] > dmax) dmax = a.sh.dims[l];
+  }
+  for (int l = 0; l < nl; ++l) {
+    fp[l] = nullptr;
+    if (l > 0 && l < nl - 1) {
+      fp[l] = p;
+      p += a.TA * a.sh.dims[l];
+    }
+  }
+  T* del0 = p;
+  T* del1 = p + a.TA * dmax;
+
+  load_input(a, acts[0], cnt, start);
+  __syncthreads();
+  forward(a, P, act, acts, fp);
+  const int nout = a.sh.dims[nl - 1];
+  for (int idx = threadIdx.x; idx < cnt * nout; idx += blockDim.x) {
+    const int at = idx % cnt, o = idx / cnt;
+    a.Eatom[(long long)a.perm[start + at] * nout + o] = (double)acts[nl - 1][at * nout + o];
+  }
+  if (!GRADIN) return;
+  for (int t = 0; t < nout; ++t) {
+    // reverse sweep for output t: delta = e_t at the (linear) output layer
+    T* dcur = del0;
+    T* dnext = del1;
+    for (int idx = threadIdx.x; idx < a.TA * nout; idx += blockDim.x) {
+      const int at = idx % a.TA, o = idx / a.TA;
+      dcur[at * nout + o] = (o == t) ? (T)1 : (T)0;
+    }
+    __syncthreads();
+    for (int l = nl - 1; l >= 1; --l) {
+      const int din = a.sh.dims[l - 1], dout = a.sh.dims[l];
+      const T* W = P + a.sh.woff[l - 1];
+      for (int idx = threadIdx.x; idx < a.TA * din; idx += blockDim.x) {
+        const int at = idx % a.TA, k = idx / a.TA;
+        T s = 0;
+        for (int o = 0; o < dout; ++o) s += W[(long long)o * din + k] * dcur[at * dout + o];
+        if (l - 1 >= 1) s *= fp[l - 1][at * din + k];
+        dnext[at * din + k] = s;
+      }
+      __syncthreads();
+      T* tmp = dcur;
+      dcur = dnext;
+      dnext = tmp;
+    }
+    // dcur = dOut_t/dx  [TA][d0]; only the ACSF part feeds the force adjoint
+    for (int idx = threadIdx.x; idx < cnt * a.nG; idx += blockDim.x) {
+      const int at = idx % cnt, f = idx / cnt;
+      const long long atom = a.perm[start + at];
+      ((T*)a.cG)[(atom * a.nout + t) * a.nG + f] = dcur[at * a.sh.dims[0] + f] * (T)a.zinv[f];
+    }
+    __syncthreads();
+  }
+}
+
+// Weight gradient of one tile (bprop), reduced over the tile in LDS and added in FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = a.first_tile + blockIdx.x;
+  const int sp = a.tile_sp[tile], start = a.tile_start[tile], cnt = a.tile_cnt[tile];
+  const int nl = a.sh.nl;
+  const T* P = (const T*)a.params + (long long)sp * a.sh.nparams;
+  double* g = a.grad + (long long)sp * a.sh.nparams;
+  const int act = a.sh.act[sp];
+  T* acts[kMaxLayers];
+  T* fp[kMaxLayers];
+  T* p = (T*)smem;
+  int dmax = 0;
+  for (int l = 0; l < nl; ++l) {
+    acts[l] = p;
+    p += a.TA * a.sh.dims[l];
+    if (a.sh.dims[l] > dmax) dmax = a.sh.dims[l];
+  }
+  for (int l = 0; l < nl; ++l) {
+    fp[l] = nullptr;
+    if (l > 0 && l < nl - 1) {
+      fp[l] = p;
+      p += a.TA * a.sh.dims[l];
+    }
+  }
+  T* dcur = p;
+  T* dnext = p + a.TA * dmax;
+  load_input(a, acts[0], cnt, start);
+  __syncthreads();
+  forward(a, P, act, acts, fp);
+  const int nout = a.sh.dims[nl - 1];
+  for (int idx = threadIdx.x; idx < a.TA * nout; idx += blockDim.x) {
+    const int at = idx % a.TA, o = idx / a.TA;
+    dcur[at * nout + o] =
+        (at < cnt) ? (T)a.lossgrad[(long long)a.perm[start + at] * nout + o] : (T)0;
+  }
+  __syncthreads();
+  for (int l = nl - 1; l >= 1; --l) {
+    const int din = a.sh.dims[l - 1], dout = a.sh.dims[l];
+    // db(l) and dW(l-1) = a_{l-1} delta_l^T summed over the tile
+    for (int o = threadIdx.x; o < dout; o += blockDim.x) {
+      T s = 0;
+      for (int at = 0; at < cnt; ++at) s += dcur[at * dout + o];
+      unsafeAtomicAdd(&g[a.sh.boff[l] + o], (double)s);
+    }
+    for (int idx = threadIdx.x; idx < din * dout; idx += blockDim.x) {
+      const int k = idx % din, o = idx / din;
+      T s = 0;
+      for (int at = 0; at < cnt; ++at) s += acts[l - 1][at * din + k] * dcur[at * dout + o];
+      unsafeAtomicAdd(&g[a.sh.woff[l - 1] + (long long)o * din + k], (double)s);
+    }
+    if (l == 1) break;
+    const T* W = P + a.sh.woff[l - 1];
+    for (int idx = threadIdx.x; idx < a.TA * din; idx += blockDim.x) {
+      const int at = idx % a.TA, k = idx / a.TA;
+      T s = 0;
+      for (int o = 0; o < dout; ++o) s += W[(long long)o * din + k] * dcur[at * dout + o];
+      dnext[at * din + k] = s * fp[l - 1][at * din + k];
+    }
+    __syncthreads();
+    T* tmp = dcur;
+    dcur = dnext;
+    dnext = tmp;
+  }
+}
+
+// One wave per structure: deterministic FP64 sum of the atomic outputs (predictBatch sums,
+// bpnn.F90:1040-1047; forces.F90:381-385).
+__global__ void __launch_bounds__(256) k_energy_reduce(const double* __restrict__ Eatom,
+                                                      const long long* __restrict__ off, int s0,
+                                                      int s1, int nout,
+                                                      double* __restrict__ Estruct) {
+  const int lane = threadIdx.x & 63;
+  const int s = s0 + (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  if (s >= s1) return;
+  const long long a0 = off[s], a1 = off[s + 1];
+  for (int t = 0; t < nout; ++t) {
+    double v = 0.0;
+    for (long long i = a0 + lane; i < a1; i += 64) v += Eatom[i * nout + t];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (lane == 0) Estruct[(long long)s * nout + t] = v;
+  }
+}
+
+// Loss gradient of every atomic output with the sysTrain scaling (bpnn.F90:677-700) and the
+// datapoint weight / normalisation of updateGradients + update (bpnn.F90:445-448,766).
+__global__ void k_lossgrad(const double* __restrict__ Eatom, const double* __restrict__ Estruct,
+                           const int* __restrict__ atom_struct, const long long* __restrict__ off,
+                           long long N, int nout, int nglob, const double* __restrict__ gt,
+                           const double* __restrict__ at, const double* __restrict__ aw,
+                           const double* __restrict__ dpw, double inv_wsum,
+                           double* __restrict__ lg) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int s = atom_struct[i];
+  const double natom = (double)(off[s + 1] - off[s]);
+  const double sc = (aw ? aw[i] : 1.0) / natom * (dpw ? dpw[s] : 1.0) * inv_wsum;
+  const int natomic = nout - nglob;
+  for (int t = 0; t < nout; ++t) {
+    double v;
+    if (t < nglob) v = 2.0 * (Estruct[(long long)s * nout + t] - gt[(long long)s * nglob + t]);
+    else v = 2.0 * (Eatom[i * nout + t] - at[i * natomic + (t - nglob)]);
+    lg[i * nout + t] = v * sc;
+  }
+}
+
+// Steepest descent of one species' parameter vector (steepdesc.F90:160-204), FP64.
+__global__ void __launch_bounds__(256) k_sd_step(double* __restrict__ x,
+                                                 const double* __restrict__ g, long long n,
+                                                 double lr, double maxdisp, double tol,
+                                                 double* __restrict__ gmax_out) {
+  __shared__ double red[256];
+  double* xs = x + (long long)blockIdx.x * n;
+  const double* gs = g + (long long)blockIdx.x * n;
+  double m = 0.0;
+  for (long long k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, fabs(gs[k]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  const double gmax = red[0];
+  const double stepmax = lr * gmax;  // max |-lr * g|
+  const double f = (stepmax <= maxdisp) ? 1.0 : maxdisp / stepmax;
+  if (threadIdx.x == 0) gmax_out[blockIdx.x] = gmax;
+  if (gmax < tol) return;  // converged: xNew = xOld
+  for (long long k = threadIdx.x; k < n; k += blockDim.x) {
+    const double step = -1.0 * lr * gs[k];
+    xs[k] = (f == 1.0) ? xs[k] + step : xs[k] + f * step;
+  }
+}
+
+__global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+template <typename T>
+size_t host_lds(const NetShape& sh, int TA) {
+  size_t s = 0;
+  int dmax = 0;
+  for (int l = 0; l < sh.nl; ++l) {
+    s += sh.dims[l];
+    if (l > 0 && l < sh.nl - 1) s += sh.dims[l];
+    if (sh.dims[l] > dmax) dmax = sh.dims[l];
+  }
+  s += 2 * dmax;
+  return s * TA * sizeof(T);
+}
+
+template <typename T>
+MlpArgs make_args(fnet_batch* b, const DevModel& m, int first_tile) {
+  MlpArgs a;
+  a.sh = m.shape;
+  a.params = std::is_same<T, double>::value ? (const void*)m.params : (const void*)m.params_f;
+  a.G = b->G;
+  a.ext = b->ext;
+  a.next = b->next;
+  a.ext_in = m.ext_in;
+  a.nG = m.nG;
+  a.next_in = m.next_in;
+  a.nout = m.nout;
+  a.zmean = m.zmean;
+  a.zinv = m.zinv;
+  a.perm = b->perm;
+  a.tile_sp = b->tile_sp;
+  a.tile_start = b->tile_start;
+  a.tile_cnt = b->tile_cnt;
+  a.first_tile = first_tile;
+  a.TA = b->tile_atoms;
+  a.Eatom = b->Eatom;
+  a.cG = b->cG;
+  a.lossgrad = b->lossgrad;
+  a.grad = nullptr;
+  return a;
+}
+
+}  // namespace
+
+int mlp_tile_atoms(const NetShape& sh, int fp64) {
+  int TA = 64;
+  while (TA > 1 && (fp64 ? host_lds<double>(sh, TA) : host_lds<float>(sh, TA)) > 64 * 1024) TA >>= 1;
+  return TA;
+}
+
+hipError_t launch_mlp_fwd(fnet_batch* b, const DevModel& m, int first_tile, int ntiles, int fp64,
+                          int want_grad_in, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  if (fp64) {
+    MlpArgs a = make_args<double>(b, m, first_tile);
+    const size_t lds = host_lds<double>(m.shape, a.TA);
+    if (want_grad_in) k_mlp_fwd<double, true><<<ntiles, 256, lds, s>>>(a);
+    else k_mlp_fwd<double, false><<<ntiles, 256, lds, s>>>(a);
+  } else {
+    MlpArgs a = make_args<float>(b, m, first_tile);
+    const size_t lds = host_lds<float>(m.shape, a.TA);
+    if (want_grad_in) k_mlp_fwd<float, true><<<ntiles, 256, lds, s>>>(a);
+    else k_mlp_fwd<float, false><<<ntiles, 256, lds, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1, hipStream_t s) {
+  const int n = s1 - s0;
+  if (n <= 0) return hipSuccess;
+  k_energy_reduce<<<(n + 3) / 4, 256, 0, s>>>(b->Eatom, b->off, s0, s1, m.nout, b->Estruct);
+  return hipGetLastError();
+}
+
+hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
+                           const double* at, const double* aw, const double* dpw, double inv_wsum,
+                           hipStream_t s) {
+  if (b->N == 0) return hipSuccess;
+  k_lossgrad<<<(unsigned)((b->N + 255) / 256), 256, 0, s>>>(b->Eatom, b->Estruct, b->atom_struct,
+                                                            b->off, b->N, m.nout, n_global, gt, at,
+                                                            aw, dpw, inv_wsum, b->lossgrad);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad,
+                           hipStream_t s) {
+  if (b->ntiles <= 0) return hipSuccess;
+  if (fp64) {
+    MlpArgs a = make_args<double>(b, m, 0);
+    a.grad = grad;
+    k_mlp_grad<double><<<b->ntiles, 256, host_lds<double>(m.shape, a.TA), s>>>(a);
+  } else {
+    MlpArgs a = make_args<float>(b, m, 0);
+    a.grad = grad;
+    k_mlp_grad<float><<<b->ntiles, 256, host_lds<float>(m.shape, a.TA), s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sd_step(double* params, const double* grad, long long n, double lr,
+                          double max_disp, double tol, double* scratch, int nspecies,
+                          hipStream_t s) {
+  k_sd_step<<<nspecies, 256, 0, s>>>(params, grad, n, lr, max_disp, tol, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_float(const double* in, float* out, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_to_float<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace fnet

@@ -1,0 +1,330 @@
+// Periodic neighbour lists for gfx950: one wavefront per centre atom, candidates tested 64 at a
+// time in the reference's entry order and compacted with a ballot prefix count, so the CSR rows
+// come out in exactly the order of dynneighlist.F90:233-320 without any sort.
+//
+// The distance test is the reference's FP64 arithmetic with FMA contraction disabled:
+//   neighCoords = coords0(:,j) + latVecs . n          (dynneighlist.F90:291, latpointiter order)
+//   dist2 = sum((coordsAtom1 - neighCoords)**2) <= cutoff**2   (dynneighlist.F90:292-294)
+// so the set of (centre, neighbour, image) entries is bit-identical to the reference's.
+//
+// Whole lattice images are skipped when no atom of the structure can reach the centre from
+// them: |row_a(inv L) . dr| <= rc |row_a(inv L)| bounds every fractional offset of a neighbour,
+// so an image n with n_a outside [s_a(i) - smax_a - w_a, s_a(i) - smin_a + w_a] holds none.
+// The exact test still decides every candidate that is looked at.
+#include "fnet_internal.hpp"
+
+namespace fnet {
+namespace {
+
+__device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
+  return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Inverse lattice and image ranges, mirroring simplealgebra.F90:41-62,90-125 and
+// latpointiter.F90:226-253 (extension 1, dynneighlist.F90:223-224).
+__device__ void lattice_setup(const double* L, double rc, StructInfo& si) {
+#pragma clang fp contract(off)
+#define O(r, c) L[((c) - 1) * 3 + ((r) - 1)]
+  double det = O(1, 1) * (O(2, 2) * O(3, 3) - O(3, 2) * O(2, 3));
+  det = det - O(1, 2) * (O(2, 1) * O(3, 3) - O(3, 1) * O(2, 3));
+  det = det + O(1, 3) * (O(2, 1) * O(3, 2) - O(3, 1) * O(2, 2));
+  double* I = si.inv;
+#define IV(r, c) I[((c) - 1) * 3 + ((r) - 1)]
+  IV(1, 1) = -O(2, 3) * O(3, 2) + O(2, 2) * O(3, 3);
+  IV(2, 1) = O(2, 3) * O(3, 1) - O(2, 1) * O(3, 3);
+  IV(3, 1) = -O(2, 2) * O(3, 1) + O(2, 1) * O(3, 2);
+  IV(1, 2) = O(1, 3) * O(3, 2) - O(1, 2) * O(3, 3);
+  IV(2, 2) = -O(1, 3) * O(3, 1) + O(1, 1) * O(3, 3);
+  IV(3, 2) = O(1, 2) * O(3, 1) - O(1, 1) * O(3, 2);
+  IV(1, 3) = -O(1, 3) * O(2, 2) + O(1, 2) * O(2, 3);
+  IV(2, 3) = O(1, 3) * O(2, 1) - O(1, 1) * O(2, 3);
+  IV(3, 3) = -O(1, 2) * O(2, 1) + O(1, 1) * O(2, 2);
+#undef IV
+#undef O
+  for (int q = 0; q < 9; ++q) I[q] = I[q] / det;
+  for (int a = 0; a < 3; ++a) {
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double v = I[c * 3 + a];
+      s = s + v * v;
+    }
+    double nrm = sqrt(s);
+    int t = (int)floor(rc * nrm);
+    si.lo[a] = -(t + 1);
+    si.hi[a] = t + 1;
+    si.w[a] = rc * nrm;
+  }
+}
+
+// One block per structure: lattice set-up by thread 0, extent of fractional coordinates by all.
+__global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ xyz,
+                                                     const double* __restrict__ lat,
+                                                     const int* __restrict__ periodic,
+                                                     const long long* __restrict__ off, double rc,
+                                                     StructInfo* __restrict__ sinfo) {
+  __shared__ StructInfo si;
+  __shared__ double red[2][3][256];
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    si.periodic = periodic[s];
+    if (si.periodic) {
+      lattice_setup(lat + 9 * s, rc, si);
+    } else {
+      for (int q = 0; q < 9; ++q) si.inv[q] = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        si.lo[a] = 0;
+        si.hi[a] = -1;
+        si.w[a] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  if (si.periodic) {
+    const long long a0 = off[s], a1 = off[s + 1];
+    for (long long i = a0 + tid; i < a1; i += blockDim.x) {
+      const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+      for (int a = 0; a < 3; ++a) {
+        double f = si.inv[0 * 3 + a] * x + si.inv[1 * 3 + a] * y + si.inv[2 * 3 + a] * z;
+        mn[a] = fmin(mn[a], f);
+        mx[a] = fmax(mx[a], f);
+      }
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    red[0][a][tid] = mn[a];
+    red[1][a][tid] = mx[a];
+  }
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      for (int a = 0; a < 3; ++a) {
+        red[0][a][tid] = fmin(red[0][a][tid], red[0][a][tid + st]);
+        red[1][a][tid] = fmax(red[1][a][tid], red[1][a][tid + st]);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      si.smin[a] = red[0][a][0];
+      si.smax[a] = red[1][a][0];
+    }
+    sinfo[s] = si;
+  }
+}
+
+// Candidate test in the reference's arithmetic.  Returns the squared distance through d2.
+__device__ __forceinline__ bool near_test(double ci0, double ci1, double ci2, const double* X,
+                                          int j, double c0, double c1, double c2, double rc2,
+                                          double& nd0, double& nd1, double& nd2, double& d2) {
+#pragma clang fp contract(off)
+  const double nc0 = X[3 * j] + c0, nc1 = X[3 * j + 1] + c1, nc2 = X[3 * j + 2] + c2;
+  const double e0 = ci0 - nc0, e1 = ci1 - nc1, e2 = ci2 - nc2;
+  double s = e0 * e0;
+  s = s + e1 * e1;
+  s = s + e2 * e2;
+  nd0 = -e0;
+  nd1 = -e1;
+  nd2 = -e2;
+  d2 = s;
+  return s <= rc2;
+}
+
+template <bool WRITE, typename V>
+__global__ void __launch_bounds__(256)
+    k_neigh(const double* __restrict__ xyz, const long long* __restrict__ off,
+            const int* __restrict__ atom_struct, const StructInfo* __restrict__ sinfo,
+            const double* __restrict__ lat, double rc, long long a0, long long a1,
+            int* __restrict__ cnt, int* __restrict__ maxnb, const int* __restrict__ rowptr,
+            int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const long long i = a0 + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  if (i >= a1) return;
+  const int s = atom_struct[i];
+  const long long o = off[s];
+  const int n = (int)(off[s + 1] - o);
+  const int li = (int)(i - o);
+  const double* X = xyz + 3 * o;
+  const double ci0 = X[3 * li], ci1 = X[3 * li + 1], ci2 = X[3 * li + 2];
+  const double rc2 = rc * rc;
+  const long long base = WRITE ? rowptr[i] : 0;
+  int count = 0;
+
+  auto emit = [&](bool hit, int j, int img, double nd0, double nd1, double nd2, double d2) {
+    const unsigned long long mask = __ballot(hit);
+    if (WRITE && hit) {
+      const long long pos = base + count + __popcll(mask & lanemask_lt(lane));
+      nj[pos] = (int)(o + j);
+      nimg[pos] = img;
+      V v;
+      v.x = nd0;
+      v.y = nd1;
+      v.z = nd2;
+      v.w = sqrt(d2);
+      nd[pos] = v;
+    }
+    count += __popcll(mask);
+  };
+
+  // central cell: j = i+1 .. n-1, 0 .. i-1
+  for (int t0 = 1; t0 < n; t0 += 64) {
+    const int t = t0 + lane;
+    int j = li + t;
+    if (j >= n) j -= n;
+    double nd0 = 0, nd1 = 0, nd2 = 0, d2 = 0;
+    bool hit = false;
+    if (t < n) hit = near_test(ci0, ci1, ci2, X, j, 0.0, 0.0, 0.0, rc2, nd0, nd1, nd2, d2);
+    emit(hit, j, -1, nd0, nd1, nd2, d2);
+  }
+  const StructInfo& si = sinfo[s];
+  if (si.periodic) {
+    const double* L = lat + 9 * s;
+    double sf[3];
+    for (int a = 0; a < 3; ++a)
+      sf[a] = si.inv[0 * 3 + a] * ci0 + si.inv[1 * 3 + a] * ci1 + si.inv[2 * 3 + a] * ci2;
+    double wlo[3], whi[3];
+    for (int a = 0; a < 3; ++a) {
+      const double m = 1e-8 * (1.0 + fabs(sf[a]) + fabs(si.smin[a]) + fabs(si.smax[a]) + si.w[a]);
+      wlo[a] = sf[a] - si.smax[a] - si.w[a] - m;
+      whi[a] = sf[a] - si.smin[a] + si.w[a] + m;
+    }
+    int img = 0;
+    for (int p1 = si.lo[0]; p1 <= si.hi[0]; ++p1)
+      for (int p2 = si.lo[1]; p2 <= si.hi[1]; ++p2)
+        for (int p3 = si.lo[2]; p3 <= si.hi[2]; ++p3) {
+          if (p1 == 0 && p2 == 0 && p3 == 0) continue;
+          const int me = img++;
+          if (p1 < wlo[0] || p1 > whi[0] || p2 < wlo[1] || p2 > whi[1] || p3 < wlo[2] ||
+              p3 > whi[2])
+            continue;
+          const double pv0 = (double)p1, pv1 = (double)p2, pv2 = (double)p3;
+          double cv[3];
+          for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            acc = acc + L[0 * 3 + r] * pv0;
+            acc = acc + L[1 * 3 + r] * pv1;
+            acc = acc + L[2 * 3 + r] * pv2;
+            cv[r] = acc;
+          }
+          for (int t0 = 0; t0 < n; t0 += 64) {
+            const int t = t0 + lane;
+            int j = li + t;
+            if (j >= n) j -= n;
+            double nd0 = 0, nd1 = 0, nd2 = 0, d2 = 0;
+            bool hit = false;
+            if (t < n)
+              hit = near_test(ci0, ci1, ci2, X, j, cv[0], cv[1], cv[2], rc2, nd0, nd1, nd2, d2);
+            emit(hit, j, me, nd0, nd1, nd2, d2);
+          }
+        }
+  }
+  if (!WRITE && lane == 0) {
+    cnt[i] = count;
+    atomicMax(maxnb, count);
+  }
+}
+
+// ---- exclusive scan of the per-atom counts -------------------------------------------------
+constexpr int kScanBlock = 1024;
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict__ in,
+                                                          int* __restrict__ out, long long n,
+                                                          int* __restrict__ totals) {
+  __shared__ int buf[kScanBlock];
+  const long long g = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+  const int v = (g < n) ? in[g] : 0;
+  buf[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = 1; st < kScanBlock; st <<= 1) {
+    int add = (threadIdx.x >= st) ? buf[threadIdx.x - st] : 0;
+    __syncthreads();
+    buf[threadIdx.x] += add;
+    __syncthreads();
+  }
+  if (g < n) out[g] = buf[threadIdx.x] - v;
+  if (threadIdx.x == kScanBlock - 1) totals[blockIdx.x] = buf[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* __restrict__ totals,
+                                                           long long nb, int* __restrict__ last) {
+  __shared__ int buf[kScanBlock];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (long long b0 = 0; b0 < nb; b0 += kScanBlock) {
+    const long long g = b0 + threadIdx.x;
+    const int v = (g < nb) ? totals[g] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 1; st < kScanBlock; st <<= 1) {
+      int add = (threadIdx.x >= st) ? buf[threadIdx.x - st] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (g < nb) totals[g] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += buf[kScanBlock - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *last = carry;
+}
+
+__global__ void k_scan_add(int* __restrict__ out, long long n, const int* __restrict__ totals) {
+  const long long g = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+  if (g < n) out[g] += totals[blockIdx.x];
+}
+
+}  // namespace
+
+long long scan_tmp_size(long long n) { return (n + kScanBlock - 1) / kScanBlock + 1; }
+
+hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s) {
+  const long long nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb == 0) {
+    return hipMemsetAsync(rowptr, 0, sizeof(int), s);
+  }
+  k_scan_block<<<(unsigned)nb, kScanBlock, 0, s>>>(cnt, rowptr, n, tmp);
+  k_scan_totals<<<1, kScanBlock, 0, s>>>(tmp, nb, rowptr + n);
+  k_scan_add<<<(unsigned)nb, kScanBlock, 0, s>>>(rowptr, n, tmp);
+  return hipGetLastError();
+}
+
+hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s) {
+  if (b->S == 0) return hipSuccess;
+  k_struct_prep<<<b->S, 256, 0, s>>>(b->xyz, b->lat, b->periodic, b->off, rc, b->sinfo);
+  return hipGetLastError();
+}
+
+hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long a1,
+                              hipStream_t s) {
+  const long long n = a1 - a0;
+  if (n <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((n + 3) / 4);
+  k_neigh<false, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
+                                               rc, a0, a1, b->cnt, b->d_maxnb, nullptr, nullptr,
+                                               nullptr, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
+                             hipStream_t s) {
+  const long long n = a1 - a0;
+  if (n <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((n + 3) / 4);
+  if (fp64) {
+    k_neigh<true, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
+                                                rc, a0, a1, nullptr, nullptr, b->rowptr, b->nj,
+                                                b->nimg, (double4*)b->nd);
+  } else {
+    k_neigh<true, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
+                                               rc, a0, a1, nullptr, nullptr, b->rowptr, b->nj,
+                                               b->nimg, (float4*)b->nd);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fnet

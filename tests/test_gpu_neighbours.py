@@ -1,0 +1,81 @@
+"""Neighbour lists on the GPU are bit-exact with the reference's iterator.
+
+The oracle restates ``TNeighIterator_getNextNeighbours`` (dynneighlist.F90:233-320) with the
+reference's FP64 operation order; the GPU list must have the same entries -- neighbour index and
+lattice image -- in the same order for every centre, including small cells where atoms meet
+their own images and several images of one neighbour.
+"""
+import numpy as np
+import pytest
+
+import fortnet_amd as fa
+from fortnet_amd import dataset
+from fortnet_amd import workloads
+from conftest import load_case
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = fa.Context(0)
+    yield c
+    c.close()
+
+
+def _model(rc, species_z=(14,)):
+    g = fa.GFunctions([2], [rc], eta=[0.1])
+    dims = [1, 2, 1]
+    nsp = len(species_z)
+    return fa.Netstat(g, species_z, dims, 0, np.zeros((nsp, fa.nparams(dims))))
+
+
+def _compare(ctx, geos, rc):
+    ctx.upload_model(_model(rc, sorted(set(int(z) for g in geos for z in g.atnum))))
+    for prec in (fa.FNET_FP64, fa.FNET_FP32):
+        b = ctx.batch(geos, prec)
+        P = b.neighbour_list()
+        rp, j, img = b.neighbours()
+        off = 0
+        for g in geos:
+            st = O.Structure(g.coords, g.latvecs, g.periodic, g.atnum)
+            for i in range(g.n):
+                oj, oimg, _, _ = O.neighbours(st, rc, i)
+                a = off + i
+                gj = j[rp[a]:rp[a + 1]] - off
+                gi = img[rp[a]:rp[a + 1]]
+                assert np.array_equal(gj, oj), (i, gj[:10], oj[:10])
+                assert np.array_equal(gi, oimg), i
+            off += g.n
+        assert P == rp[-1]
+        b.close()
+
+
+def test_fixture_cells_exact(ctx):
+    c, ds = load_case("analysis__derivatives__analytical__singleTarget__speciesResolved__raw")
+    geos = dataset.geometries(ds)
+    _compare(ctx, geos, float(c["in_g_rcut"].max()))
+
+
+def test_cluster_exact(ctx):
+    c, ds = load_case("validation__multispecies__bothTargets")
+    _compare(ctx, dataset.geometries(ds)[:4], 7.5589039)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_small_cells_with_self_images(ctx, seed):
+    """8..40-atom cells shorter than the cutoff (many images per neighbour, self images)."""
+    rng = np.random.default_rng(seed)
+    geos = []
+    for n in (1, 2, 8, 13, 40):
+        geos.append(workloads.random_cell(rng, n, 2.33, 2.0, 14))
+    _compare(ctx, geos, 6.0 * fa.AA__BOHR)
+
+
+def test_triclinic_and_unwrapped(ctx):
+    rng = np.random.default_rng(7)
+    L = np.array([[9.0, 0.0, 0.0], [2.5, 8.0, 0.0], [-1.5, 3.0, 10.0]])
+    xyz = rng.uniform(-0.3, 1.3, size=(30, 3)) @ L  # some atoms outside the cell
+    g = fa.Geometry(xyz, np.full(30, 14), np.zeros(30, np.int32), L)
+    _compare(ctx, [g], 7.0)

@@ -1,0 +1,142 @@
+"""GPU parity on the synthetic workloads (small instances) against the FP64 oracle.
+
+* FP64 path: ACSF values, energies and forces within 1e-10 (relative to the largest value);
+* FP32 path: the accuracy gates of the batched mixed-precision path -- energy error
+  <= 1e-5 eV/atom and force RMSE <= 1e-4 eV/Angstrom (network outputs are read as eV, lengths
+  are Bohr, so forces are converted with 1 eV/Bohr = 1/0.529177249 eV/Angstrom);
+* per-structure launches and the batched launch give the same numbers;
+* training gradients: FP64 within 1e-9 relative, FP32 within 1e-3 relative (stated, not gated
+  by the reference).
+"""
+import numpy as np
+import pytest
+
+import fortnet_amd as fa
+from fortnet_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-5            # eV/atom
+F_RMSE_TOL = 1e-4       # eV/Angstrom
+EV_BOHR_TO_EV_AA = 1.0 / fa.BOHR__AA
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = fa.Context(0)
+    yield c
+    c.close()
+
+
+def _oracle_all(geos, ns):
+    """Oracle G, atomic outputs, forces for a list of geometries."""
+    gf = O.GFunctions(ns.g.type, ns.g.rcut, ns.g.eta, ns.g.rs, ns.g.kappa, ns.g.lam, ns.g.xi,
+                      ns.g.atomid, ns.g.z1, ns.g.z2)
+    nets = O.Nets(ns.dims, ns.activation, ns.params)
+    Gs, Ea, Fs = [], [], []
+    for g in geos:
+        st = O.Structure(g.coords, g.latvecs, int(g.periodic), g.atnum, g.ext)
+        G = O.acsf(st, gf)
+        X = O.zscore(G, ns.zmean, ns.zstd)
+        out = nets.predict(X, g.species)
+        J = nets.jacobian(X, g.species)
+        sc = O.zscale(ns.zstd, gf.n)
+        F = O.acsf_adjoint(st, gf, J[:, 0, :] * sc[None, :])
+        Gs.append(G)
+        Ea.append(out)
+        Fs.append(F)
+    return np.concatenate(Gs), np.concatenate(Ea), np.concatenate(Fs)
+
+
+CASES = {
+    "si64": ("si64", 0.02),
+    "water": ("water", 0.004),
+    "si_cells": ("si_cells", 0.0015),
+    "si_bulk": ("si_bulk", 0.125),
+    "ternary": ("ternary", 0.016),
+}
+
+
+def _setup(ctx, name):
+    cfg, scale = CASES[name]
+    geos, ns, _ = W.config(cfg, scale)
+    if name == "ternary":
+        geos = [g for g in geos if g.n <= 400][:2] or geos[:1]
+    ctx.upload_model(ns)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    ns = W.with_zscore(ns, b.acsf())
+    b.close()
+    ctx.upload_model(ns)
+    return geos, ns
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_matches_oracle(ctx, name):
+    print("workload", name, flush=True)
+    geos, ns = _setup(ctx, name)
+    G0, E0, F0 = _oracle_all(geos, ns)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    G = b.acsf()
+    Es, Ea = b.energy()
+    F = b.forces()
+    assert np.abs(G - G0).max() <= 1e-10 * max(1.0, np.abs(G0).max())
+    assert np.abs(Ea - E0).max() <= 1e-10 * max(1.0, np.abs(E0).max())
+    assert np.abs(F - F0).max() <= 1e-9 * max(1.0, np.abs(F0).max())
+    off = np.concatenate([[0], np.cumsum([g.n for g in geos])])
+    Es0 = np.array([E0[off[s]:off[s + 1]].sum() for s in range(len(geos))])
+    assert np.abs(Es[:, 0] - Es0).max() <= 1e-10 * max(1.0, np.abs(Es0).max())
+    # per-structure launches (the plain path) give the same numbers
+    b.set_mode(fa.FNET_FP64, fa.FNET_PER_STRUCT)
+    Es2, Ea2 = b.energy()
+    F2 = b.forces()
+    assert np.abs(Ea2 - Ea).max() <= 1e-12 * max(1.0, np.abs(Ea).max())
+    assert np.abs(F2 - F).max() <= 1e-11 * max(1.0, np.abs(F).max())
+    b.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp32_meets_accuracy_gates(ctx, name):
+    print("workload", name, flush=True)
+    geos, ns = _setup(ctx, name)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    Es64, _ = b.energy()
+    F64 = b.forces()
+    b.set_mode(fa.FNET_FP32)
+    Es32, _ = b.energy()
+    F32 = b.forces()
+    nat = np.array([g.n for g in geos])
+    e_err = np.abs(Es32[:, 0] - Es64[:, 0]) / nat
+    f_rmse = np.sqrt(np.mean((F32 - F64) ** 2)) * EV_BOHR_TO_EV_AA
+    print("%s: max |dE|/atom %.3e eV, force RMSE %.3e eV/A" % (name, e_err.max(), f_rmse))
+    assert e_err.max() <= E_TOL
+    assert f_rmse <= F_RMSE_TOL
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["si64", "water"])
+def test_train_gradients_match_oracle(ctx, name):
+    print("workload", name, flush=True)
+    geos, ns = _setup(ctx, name)
+    gf = O.GFunctions(ns.g.type, ns.g.rcut, ns.g.eta, ns.g.rs, ns.g.kappa, ns.g.lam, ns.g.xi,
+                      ns.g.atomid, ns.g.z1, ns.g.z2)
+    nets = O.Nets(ns.dims, ns.activation, ns.params)
+    rng = np.random.default_rng(0)
+    targets = rng.normal(size=(len(geos), 1)) * 3.0
+    weights = rng.integers(1, 4, size=len(geos)).astype(np.float64)
+    aw = np.concatenate([rng.uniform(0.5, 1.5, size=g.n) for g in geos])
+    grad0 = np.zeros_like(ns.params)
+    off = 0
+    for s, g in enumerate(geos):
+        st = O.Structure(g.coords, g.latvecs, int(g.periodic), g.atnum)
+        X = O.zscore(O.acsf(st, gf), ns.zmean, ns.zstd)
+        nets.sys_train(X, g.species, 1, targets[s], None, aw[off:off + g.n], weights[s], grad0)
+        off += g.n
+    grad0 /= weights.sum()
+    for prec, tol in ((fa.FNET_FP64, 1e-9), (fa.FNET_FP32, 1e-3)):
+        b = ctx.batch(geos, prec)
+        grad, _ = b.train_gradients(targets, None, aw, weights)
+        rel = np.abs(grad - grad0).max() / np.abs(grad0).max()
+        print("%s precision %d: gradient max rel err %.3e" % (name, prec, rel))
+        assert rel <= tol
+        b.close()
