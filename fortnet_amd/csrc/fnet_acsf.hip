@@ -1,22 +1,28 @@
 // ACSF values and their adjoint (force) pass on gfx950.
 //
 // One wavefront per centre atom.  The centre's CSR row of the neighbour list (image displacement
-// and distance per entry) is staged once into the wave's LDS slice with coalesced loads, and
-// per-element index lists are built with ballots, so every species-resolved list of
-// buildGFunctionNeighborlists (acsf.F90:943-1066) is a slice of LDS instead of a rebuilt
-// neighbour list.
+// and distance per entry) is staged once into the wave's LDS slice with coalesced loads, with
+// the cutoff function of the list cutoff and per-element index lists built by ballots, so every
+// species-resolved list of buildGFunctionNeighborlists (acsf.F90:943-1066) is a slice of LDS
+// instead of a rebuilt neighbour list.
 //
-//  * radial G1/G2/G3 (acsf.F90:1287-1373): lanes = functions, loop over the staged entries;
-//  * angular G4/G5 (acsf.F90:1377-1492): functions that share (type, cutoff, eta, species pair,
-//    atom id) form a group; lanes stride over the neighbour pairs of the group once and update
-//    every member.  When both lists are the same (unresolved, or Z1 == Z2) the ordered double
-//    sum with its j == k diagonal is taken as 2 * (j < k) + diagonal;
+//  * radial G1/G2/G3 (acsf.F90:1287-1373): lanes = (entry stride group, function), partial sums
+//    combined through LDS;
+//  * angular G4/G5 (acsf.F90:1377-1492): functions sharing (type, cutoff, eta, species pair,
+//    atom id) form a group; lanes stride over the group's neighbour pairs once and update every
+//    member.  Pairs are enumerated by offset (k = j + d): the diagonal, then each unordered pair
+//    once with weight 2 when both lists are the same (the ordered double sum with its j == k
+//    terms, acsf.F90:1420-1432), or the full list1 x list2 product otherwise.  With this order
+//    the 64 lanes of one step touch 64 different entries, which keeps the adjoint's LDS
+//    accumulation free of same-address collisions;
 //  * the adjoint pass computes F = -dE/dR directly from c = dE/dG (the network's input gradient
-//    times the z-score factor): per entry, the derivative of every function is weighted by c and
-//    summed into one 3-vector per neighbour entry (LDS atomics), which is then scattered to the
-//    neighbour's force (-) and the centre's force (+).  This is gFuncGrad (acsf.F90:1496-1670)
-//    contracted with the Jacobian as in forces.F90:400-413, without the dense dG/dr array, and
-//    with periodic images summed exactly.
+//    times the z-score factor): the derivative of every function with respect to each entry's
+//    position (gFuncGrad, acsf.F90:1496-1670) is weighted by c and summed per entry, then scattered
+//    to the neighbour's force (-) and the centre's force (+), as forces.F90:400-413 contracts the
+//    dense array, but without forming it and with periodic images summed exactly.
+//
+// Arithmetic is T (double or float); sums over neighbours and pairs, the values G and the forces
+// are accumulated in FP64 in both modes.
 #include "fnet_internal.hpp"
 
 namespace fnet {
@@ -58,8 +64,7 @@ __device__ __forceinline__ T xpow(T x, int n, T y) {
   return r;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   return v;
@@ -82,58 +87,66 @@ struct AcsfArgs {
   double rclist;
   long long a0, a1;
   int maxnb;
-  void* G;        // [N][nG] (forward)
-  const void* cG; // [N][nout][nG] (adjoint)
-  void* F;        // [N][nout][3] (adjoint)
+  double* G;      // [N][nG] (forward)
+  const void* cG; // [N][nout][nG] T (adjoint)
+  double* F;      // [N][nout][3] (adjoint)
 };
 
-// LDS slice of one wave.
+// Bytes of one wave's LDS slice (host and device agree on this layout).
 template <typename T, bool ADJ>
-struct Slice {
-  using V = typename Vec4<T>::type;
-  V* d;          // [M] displacement (x,y,z) and distance (w)
-  int* j;        // [M] global neighbour index
-  int* zc;       // [M] element code
-  T* E;          // [M] exp(-eta r^2) fc(r) of the current group
-  T* Ep;         // [M] its r-derivative (adjoint)
-  T* q;          // [M] atom-id prefactor of the current group
-  T* gs;         // [3M] accumulated dE/d(entry position) (adjoint)
-  unsigned short* idx; // [nZ][M]
-  int* nz;       // [nZ]
-};
-
-template <typename T, bool ADJ>
-__device__ size_t slice_bytes(int M, int nZ) {
-  size_t b = sizeof(typename Vec4<T>::type) * M + 2 * sizeof(int) * M + 2 * sizeof(T) * M;
-  if (ADJ) b += sizeof(T) * M + 3 * sizeof(T) * M;
-  b += sizeof(unsigned short) * (size_t)nZ * M + sizeof(int) * kMaxZ;
+__host__ __device__ size_t slice_bytes(int M, int nZ) {
+  size_t b = sizeof(typename Vec4<T>::type) * M;  // d
+  b += 5 * sizeof(T) * M;                          // ir, fc, dfc, E, q
+  if (ADJ) b += sizeof(T) * M + 3 * sizeof(double) * M;  // Ep, gs
+  b += 2 * sizeof(int) * M;                        // j, zc
+  b += 64 * sizeof(double);                        // radial partial sums
+  b += sizeof(int) * kMaxZ;                        // nz
+  b += sizeof(unsigned short) * (size_t)nZ * M;    // idx
   return (b + 31) & ~size_t(31);
 }
 
-template <typename T, bool ADJ>
-__device__ Slice<T, ADJ> make_slice(unsigned char* base, int M, int nZ) {
+template <typename T>
+struct Slice {
   using V = typename Vec4<T>::type;
-  Slice<T, ADJ> s;
-  unsigned char* p = base;
+  V* d;
+  T *ir, *fc, *dfc, *E, *q, *Ep;
+  double* gs;
+  int *j, *zc;
+  double* red;
+  int* nz;
+  unsigned short* idx;
+};
+
+template <typename T, bool ADJ>
+__device__ Slice<T> make_slice(unsigned char* p, int M) {
+  using V = typename Vec4<T>::type;
+  Slice<T> s;
   s.d = (V*)p;
   p += sizeof(V) * M;
+  s.ir = (T*)p;
+  p += sizeof(T) * M;
+  s.fc = (T*)p;
+  p += sizeof(T) * M;
+  s.dfc = (T*)p;
+  p += sizeof(T) * M;
   s.E = (T*)p;
   p += sizeof(T) * M;
   s.q = (T*)p;
   p += sizeof(T) * M;
+  s.Ep = nullptr;
+  s.gs = nullptr;
   if (ADJ) {
     s.Ep = (T*)p;
     p += sizeof(T) * M;
-    s.gs = (T*)p;
-    p += 3 * sizeof(T) * M;
-  } else {
-    s.Ep = nullptr;
-    s.gs = nullptr;
+    s.gs = (double*)p;
+    p += 3 * sizeof(double) * M;
   }
   s.j = (int*)p;
   p += sizeof(int) * M;
   s.zc = (int*)p;
   p += sizeof(int) * M;
+  s.red = (double*)p;
+  p += 64 * sizeof(double);
   s.nz = (int*)p;
   p += sizeof(int) * kMaxZ;
   s.idx = (unsigned short*)p;
@@ -144,50 +157,67 @@ __device__ __forceinline__ double ext_q(const double* ext, int next, int atomid,
   return atomid > 0 ? ext[atom * next + (atomid - 1)] : 1.0;
 }
 
-// Enumerates the pairs of one group: same lists -> j <= k (triangle with diagonal), otherwise the
-// full rectangle list1 x list2.  Calls fn(e_j, e_k, diag).
-template <typename Fn>
-__device__ __forceinline__ void for_pairs(int lane, bool same, int L1, int L2,
-                                          const unsigned short* l1, const unsigned short* l2,
-                                          Fn&& fn) {
-  if (same) {
-    if (L1 <= 0) return;
-    int j = 0, k = lane;
-    while (j < L1 && k >= L1) {
-      k = k - L1 + j + 1;
-      ++j;
-    }
-    while (j < L1) {
-      const int ej = l1 ? l1[j] : j;
-      const int ek = l1 ? l1[k] : k;
-      fn(ej, ek, j == k);
-      k += 64;
-      while (j < L1 && k >= L1) {
-        k = k - L1 + j + 1;
-        ++j;
-      }
-    }
-  } else {
-    if (L1 <= 0 || L2 <= 0) return;
-    int j = 0, k = lane;
-    while (j < L1 && k >= L2) {
-      k -= L2;
-      ++j;
-    }
-    while (j < L1) {
-      const int ej = l1 ? l1[j] : j;
-      const int ek = l2 ? l2[k] : k;
-      fn(ej, ek, false);
-      k += 64;
-      while (j < L1 && k >= L2) {
-        k -= L2;
-        ++j;
-      }
-    }
-  }
-}
+// Pair enumeration of one group.  Same lists: the diagonal (d = 0, weight 1), then every
+// unordered pair once as (j, j + d mod L), d = 1 .. L/2, weight 2.  Different lists: every pair of
+// list1 x list2 once; the longer list runs over the lanes and k = j + d mod (shorter length).
+struct PairIter {
+  const unsigned short* la;
+  const unsigned short* lb;
+  int LA, LB, d, j, dmax;
+  bool same, even, done;
 
-template <typename T, bool ADJ>
+  __device__ __forceinline__ void init(int lane, bool s, int L1, int L2, const unsigned short* l1,
+                                       const unsigned short* l2) {
+    same = s;
+    if (same) {
+      LA = LB = L1;
+      la = lb = l1;
+    } else {
+      const bool sw = L2 > L1;
+      LA = sw ? L2 : L1;
+      LB = sw ? L1 : L2;
+      la = sw ? l2 : l1;
+      lb = sw ? l1 : l2;
+    }
+    dmax = (LA - 1) / 2;
+    even = (LA & 1) == 0;
+    d = 0;
+    j = lane;
+    done = (LA <= 0 || LB <= 0);
+    if (!done)
+      while (j >= LA) {
+        j -= LA;
+        ++d;
+      }
+  }
+
+  __device__ __forceinline__ bool next(int& ej, int& ek, bool& diag, int& w) {
+    if (done) return false;
+    if (same) {
+      if (d > dmax && !(even && d == dmax + 1 && j < (LA >> 1))) {
+        done = true;
+        return false;
+      }
+    } else if (d >= LB) {
+      done = true;
+      return false;
+    }
+    int k = j + d;
+    while (k >= LB) k -= LB;
+    ej = la ? (int)la[j] : j;
+    ek = lb ? (int)lb[k] : k;
+    diag = same && d == 0;
+    w = (same && d != 0) ? 2 : 1;
+    j += 64;
+    while (j >= LA) {
+      j -= LA;
+      ++d;
+    }
+    return true;
+  }
+};
+
+template <typename T, bool ADJ, int MQ>
 __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
@@ -196,15 +226,15 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
   const long long i = a.a0 + (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (i >= a.a1) return;
   const int M = a.maxnb;
-  const size_t sb = slice_bytes<T, ADJ>(M, a.nZ);
-  Slice<T, ADJ> sl = make_slice<T, ADJ>(smem + sb * wave, M, a.nZ);
+  Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M, a.nZ) * wave, M);
 
   const int b = a.rowptr[i];
   const int n = a.rowptr[i + 1] - b;
   const V* nd = (const V*)a.nd + b;
+  const T pi = (T)kPi;
+  const T rcl = (T)a.rclist;
 
-  // ---- stage the centre's entries and build per-element index lists ----
-  if (lane < kMaxZ) sl.nz[lane] = 0;
+  // ---- stage the centre's entries, list-cutoff fc, per-element index lists ----
   int running[kMaxZ];
 #pragma unroll
   for (int c = 0; c < kMaxZ; ++c) running[c] = 0;
@@ -213,10 +243,15 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
     int zc = -1;
     if (e < n) {
       const int jj = a.nj[b + e];
-      sl.d[e] = nd[e];
+      const V d = nd[e];
+      sl.d[e] = d;
       sl.j[e] = jj;
       zc = a.zcode[jj];
       sl.zc[e] = zc;
+      sl.ir[e] = (T)1 / d.w;
+      const T x = pi * d.w / rcl;
+      sl.fc[e] = (T)0.5 * (fcos(x) + (T)1);
+      sl.dfc[e] = -(T)0.5 * (pi / rcl) * fsin(x);
     }
 #pragma unroll
     for (int c = 0; c < kMaxZ; ++c) {
@@ -227,14 +262,11 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
       }
     }
   }
-  if (lane < a.nZ) {
 #pragma unroll
-    for (int c = 0; c < kMaxZ; ++c)
-      if (c == lane) sl.nz[c] = running[c];
-  }
+  for (int c = 0; c < kMaxZ; ++c)
+    if (c == lane) sl.nz[c] = running[c];
   wave_sync();
 
-  const T pi = (T)kPi;
   const int nouts = ADJ ? a.nout : 1;
   for (int t = 0; t < nouts; ++t) {
     const T* cG = ADJ ? (const T*)a.cG + (i * a.nout + t) * a.nG : nullptr;
@@ -242,28 +274,43 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
     // ---- radial functions ----
     if (!ADJ) {
       for (int f0 = 0; f0 < a.nR; f0 += 64) {
-        const int f = f0 + lane;
-        const bool act = f < a.nR;
-        RadialFn fn;
-        if (act) fn = a.radial[f];
-        else {
-          fn.col = 0; fn.type = 0; fn.zc = -1; fn.atomid = 0; fn.rc = 1; fn.eta = fn.rs = fn.kappa = 0;
-        }
+        const int nf = min(64, a.nR - f0);
+        const int ng = 64 / nf;
+        const int fl = lane % nf, gl = lane / nf;
+        const bool act = gl < ng;
+        const RadialFn& fn = a.radial[f0 + fl];
+        const bool shared = (fn.rc == a.rclist) && fn.atomid == 0;
         const T rc = (T)fn.rc, eta = (T)fn.eta, rs = (T)fn.rs, kap = (T)fn.kappa;
         const T qi = (T)ext_q(a.ext, a.next, fn.atomid, i);
-        T acc = 0;
-        for (int e = 0; e < n; ++e) {
-          const V d = sl.d[e];
-          const int zc = sl.zc[e];
-          if (act && (fn.zc < 0 || fn.zc == zc) && d.w <= rc) {
-            const T qj = fn.atomid > 0 ? (T)ext_q(a.ext, a.next, fn.atomid, sl.j[e]) : (T)1;
-            const T fc = (T)0.5 * qi * qj * (fcos(pi * d.w / rc) + (T)1);
-            if (fn.type == 1) acc += fc;
-            else if (fn.type == 2) acc += fexp(-eta * ((d.w - rs) * (d.w - rs))) * fc;
-            else acc += fcos(kap * d.w) * fc;
+        double acc = 0.0;
+        if (act) {
+          for (int e = gl; e < n; e += ng) {
+            const T r = sl.d[e].w;
+            const int zc = sl.zc[e];
+            if ((fn.zc < 0 || fn.zc == zc) && r <= rc) {
+              T fcv;
+              if (shared) {
+                fcv = sl.fc[e];
+              } else {
+                const T qj = (T)ext_q(a.ext, a.next, fn.atomid, sl.j[e]);
+                fcv = (T)0.5 * qi * qj * (fcos(pi * r / rc) + (T)1);
+              }
+              T term;
+              if (fn.type == 1) term = fcv;
+              else if (fn.type == 2) term = fexp(-eta * ((r - rs) * (r - rs))) * fcv;
+              else term = fcos(kap * r) * fcv;
+              acc += (double)term;
+            }
           }
         }
-        if (act) ((T*)a.G)[i * a.nG + fn.col] = acc;
+        sl.red[lane] = acc;
+        wave_sync();
+        if (lane < nf) {
+          double s = 0.0;
+          for (int g = 0; g < ng; ++g) s += sl.red[g * nf + lane];
+          a.G[i * a.nG + fn.col] = s;
+        }
+        wave_sync();
       }
     } else {
       for (int e0 = 0; e0 < n; e0 += 64) {
@@ -278,12 +325,18 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
           const T cf = cG[fn.col];
           if (cf == (T)0) continue;
           if (!(fn.zc < 0 || fn.zc == zc) || !(r <= (T)fn.rc)) continue;
-          const T rc = (T)fn.rc;
-          const T qq = (T)(ext_q(a.ext, a.next, fn.atomid, i) *
-                           (fn.atomid > 0 ? ext_q(a.ext, a.next, fn.atomid, sl.j[e]) : 1.0));
-          const T x = pi * r / rc;
-          const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
-          const T dfc = -(T)0.5 * qq * (pi / rc) * fsin(x);
+          T fc, dfc;
+          if (fn.rc == a.rclist && fn.atomid == 0) {
+            fc = sl.fc[e];
+            dfc = sl.dfc[e];
+          } else {
+            const T rc = (T)fn.rc;
+            const T qq = (T)(ext_q(a.ext, a.next, fn.atomid, i) *
+                             ext_q(a.ext, a.next, fn.atomid, sl.j[e]));
+            const T x = pi * r / rc;
+            fc = (T)0.5 * qq * (fcos(x) + (T)1);
+            dfc = -(T)0.5 * qq * (pi / rc) * fsin(x);
+          }
           T g;
           if (fn.type == 1) {
             g = dfc;
@@ -296,10 +349,10 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
           }
           s += cf * g;
         }
-        const T sr = s / r;
-        sl.gs[3 * e + 0] = sr * d.x;
-        sl.gs[3 * e + 1] = sr * d.y;
-        sl.gs[3 * e + 2] = sr * d.z;
+        const double sr = (double)(s * sl.ir[e]);
+        sl.gs[3 * e + 0] = sr * (double)d.x;
+        sl.gs[3 * e + 1] = sr * (double)d.y;
+        sl.gs[3 * e + 2] = sr * (double)d.z;
       }
       wave_sync();
     }
@@ -308,11 +361,11 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
     for (int g = 0; g < a.nAG; ++g) {
       const AngGroup& grp = a.groups[g];
       const int nm = grp.nmem;
-      T A[kMaxMembers];
-      bool any = !ADJ;
+      T A[MQ];
       if (ADJ) {
+        bool any = false;
 #pragma unroll
-        for (int q = 0; q < kMaxMembers; ++q) {
+        for (int q = 0; q < MQ; ++q) {
           A[q] = (q < nm) ? cG[grp.col[q]] * (T)grp.scale[q] : (T)0;
           any = any || (A[q] != (T)0);
         }
@@ -321,52 +374,62 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
       const T rc = (T)grp.rc, eta = (T)grp.eta;
       const bool g4 = grp.type == 4;
       const bool same = grp.same != 0;
+      const bool shared = (grp.rc == a.rclist) && grp.atomid == 0;
       const unsigned short* l1 = grp.zc1 >= 0 ? sl.idx + grp.zc1 * M : nullptr;
       const unsigned short* l2 = grp.zc2 >= 0 ? sl.idx + grp.zc2 * M : nullptr;
       const int L1 = grp.zc1 >= 0 ? sl.nz[grp.zc1] : n;
-      const int L2 = same ? L1 : (grp.zc2 >= 0 ? sl.nz[grp.zc2] : n);
+      const int L2 = grp.zc2 >= 0 ? sl.nz[grp.zc2] : n;
       const T qi = (T)ext_q(a.ext, a.next, grp.atomid, i);
-      // per-entry radial factors of this group
+      // per-entry radial factor exp(-eta r^2) fc(r) of this group (and its r-derivative)
       for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
         if (e >= n) break;
         const T r = sl.d[e].w;
-        const T qj = grp.atomid > 0 ? (T)ext_q(a.ext, a.next, grp.atomid, sl.j[e]) : (T)1;
-        sl.q[e] = qj;
-        T E = 0, Ep = 0;
+        T qj = 1, E = 0, Ep = 0;
         if (r <= rc) {
-          const T x = pi * r / rc;
-          const T ex = fexp(-eta * r * r);
-          const T fc = (T)0.5 * qi * qj * (fcos(x) + (T)1);
-          E = ex * fc;
-          if (ADJ) {
-            const T dfc = -(T)0.5 * qi * qj * (pi / rc) * fsin(x);
-            Ep = ex * (dfc - (T)2 * eta * r * fc);
+          T fc, dfc;
+          if (shared) {
+            fc = sl.fc[e];
+            dfc = sl.dfc[e];
+          } else {
+            qj = (T)ext_q(a.ext, a.next, grp.atomid, sl.j[e]);
+            const T x = pi * r / rc;
+            fc = (T)0.5 * qi * qj * (fcos(x) + (T)1);
+            dfc = -(T)0.5 * qi * qj * (pi / rc) * fsin(x);
           }
+          const T ex = fexp(-eta * r * r);
+          E = ex * fc;
+          Ep = ex * (dfc - (T)2 * eta * r * fc);
         }
+        sl.q[e] = qj;
         sl.E[e] = E;
         if (ADJ) sl.Ep[e] = Ep;
       }
       wave_sync();
 
-      int xint[kMaxMembers];
-      T xi[kMaxMembers], lam[kMaxMembers];
+      int xint[MQ];
+      T xi[MQ], lam[MQ];
 #pragma unroll
-      for (int q = 0; q < kMaxMembers; ++q) {
+      for (int q = 0; q < MQ; ++q) {
         xint[q] = (q < nm) ? grp.xint[q] : 0;
         xi[q] = (q < nm) ? (T)grp.xi[q] : (T)0;
         lam[q] = (q < nm) ? (T)grp.lam[q] : (T)0;
       }
 
       if (!ADJ) {
-        T acc[kMaxMembers];
+        double acc[MQ];
 #pragma unroll
-        for (int q = 0; q < kMaxMembers; ++q) acc[q] = 0;
-        for_pairs(lane, same, L1, L2, l1, l2, [&](int ej, int ek, bool diag) {
+        for (int q = 0; q < MQ; ++q) acc[q] = 0.0;
+        PairIter it;
+        it.init(lane, same, L1, L2, l1, l2);
+        int ej, ek, wi;
+        bool diag;
+        while (it.next(ej, ek, diag, wi)) {
+          const T w = (T)wi;
           const V dj = sl.d[ej], dk = sl.d[ek];
-          const T dot = dj.x * dk.x + dj.y * dk.y + dj.z * dk.z;
-          const T c = dot / (dj.w * dk.w + (T)1e-13);
-          T base = sl.E[ej] * sl.E[ek] * ((same && !diag) ? (T)2 : (T)1);
+          T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * sl.ir[ej] * sl.ir[ek];
+          c = fmin(fmax(c, (T)-1), (T)1);
+          T base = w * sl.E[ej] * sl.E[ek];
           if (g4) {
             const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
             const T r2 = rx * rx + ry * ry + rz * rz;
@@ -379,38 +442,43 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
             }
           }
 #pragma unroll
-          for (int q = 0; q < kMaxMembers; ++q)
-            if (q < nm) acc[q] += base * xpow((T)1 + lam[q] * c, xint[q], xi[q]);
-        });
+          for (int q = 0; q < MQ; ++q)
+            if (q < nm) acc[q] += (double)(base * xpow((T)1 + lam[q] * c, xint[q], xi[q]));
+        }
 #pragma unroll
-        for (int q = 0; q < kMaxMembers; ++q) {
+        for (int q = 0; q < MQ; ++q) {
           if (q < nm) {
-            const T s = wave_sum(acc[q]);
-            if (lane == 0) ((T*)a.G)[i * a.nG + grp.col[q]] = s * (T)grp.scale[q];
+            const double s = wave_sum(acc[q]);
+            if (lane == 0) a.G[i * a.nG + grp.col[q]] = s * grp.scale[q];
           }
         }
       } else {
         // diagonal coefficient: P = 1 + lambda at c = 1 (acsf.F90:1580-1585)
         T S0d = 0;
 #pragma unroll
-        for (int q = 0; q < kMaxMembers; ++q)
+        for (int q = 0; q < MQ; ++q)
           if (q < nm) S0d += A[q] * xpow((T)1 + lam[q], xint[q], xi[q]);
-        for_pairs(lane, same, L1, L2, l1, l2, [&](int ej, int ek, bool diag) {
+        PairIter it;
+        it.init(lane, same, L1, L2, l1, l2);
+        int ej, ek, wi;
+        bool diag;
+        while (it.next(ej, ek, diag, wi)) {
+          const T w = (T)wi;
           const V dj = sl.d[ej];
-          const T rj = dj.w;
+          const T irj = sl.ir[ej];
           const T Ej = sl.E[ej], Epj = sl.Ep[ej];
           if (diag) {
             const T H = g4 ? sl.q[ej] * sl.q[ej] : (T)1;
-            const T gsc = S0d * (T)2 * Ej * Epj * H / rj;
-            atomicAdd(&sl.gs[3 * ej + 0], gsc * dj.x);
-            atomicAdd(&sl.gs[3 * ej + 1], gsc * dj.y);
-            atomicAdd(&sl.gs[3 * ej + 2], gsc * dj.z);
-            return;
+            const T gsc = S0d * (T)2 * Ej * Epj * H * irj;
+            atomicAdd(&sl.gs[3 * ej + 0], (double)(gsc * dj.x));
+            atomicAdd(&sl.gs[3 * ej + 1], (double)(gsc * dj.y));
+            atomicAdd(&sl.gs[3 * ej + 2], (double)(gsc * dj.z));
+            continue;
           }
           const V dk = sl.d[ek];
-          const T rk = dk.w;
+          const T irk = sl.ir[ek];
           const T Ek = sl.E[ek], Epk = sl.Ep[ek];
-          T H = 1, Hp = 0, rjk = 0;
+          T H = 1, Hp = 0, rjk = 1;
           T rx = 0, ry = 0, rz = 0;
           if (g4) {
             rx = dj.x - dk.x;
@@ -418,7 +486,7 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
             rz = dj.z - dk.z;
             const T r2 = rx * rx + ry * ry + rz * rz;
             rjk = sqrt(r2);
-            if (!(rjk <= rc)) return;
+            if (!(rjk <= rc)) continue;
             const T qq = sl.q[ej] * sl.q[ek];
             const T x = pi * rjk / rc;
             const T ex = fexp(-eta * r2);
@@ -427,13 +495,12 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
             H = ex * fc;
             Hp = ex * (dfc - (T)2 * eta * rjk * fc);
           }
-          const T irj = (T)1 / rj, irk = (T)1 / rk;
           const T ujx = dj.x * irj, ujy = dj.y * irj, ujz = dj.z * irj;
           const T ukx = dk.x * irk, uky = dk.y * irk, ukz = dk.z * irk;
-          const T c = ujx * ukx + ujy * uky + ujz * ukz;
+          const T c = fmin(fmax(ujx * ukx + ujy * uky + ujz * ukz, (T)-1), (T)1);
           T S0 = 0, S1 = 0;
 #pragma unroll
-          for (int q = 0; q < kMaxMembers; ++q) {
+          for (int q = 0; q < MQ; ++q) {
             if (q < nm) {
               const T P = (T)1 + lam[q] * c;
               T pw, pm1;
@@ -451,7 +518,6 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
               S1 += A[q] * xi[q] * lam[q] * pm1;
             }
           }
-          const T w = same ? (T)2 : (T)1;
           const T B = Ej * Ek * H;
           const T cj = w * S1 * B * irj, ck = w * S1 * B * irk;
           const T aj = w * S0 * Epj * Ek * H, ak = w * S0 * Ej * Epk * H;
@@ -470,25 +536,25 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
             gky -= h * ry;
             gkz -= h * rz;
           }
-          atomicAdd(&sl.gs[3 * ej + 0], gjx);
-          atomicAdd(&sl.gs[3 * ej + 1], gjy);
-          atomicAdd(&sl.gs[3 * ej + 2], gjz);
-          atomicAdd(&sl.gs[3 * ek + 0], gkx);
-          atomicAdd(&sl.gs[3 * ek + 1], gky);
-          atomicAdd(&sl.gs[3 * ek + 2], gkz);
-        });
+          atomicAdd(&sl.gs[3 * ej + 0], (double)gjx);
+          atomicAdd(&sl.gs[3 * ej + 1], (double)gjy);
+          atomicAdd(&sl.gs[3 * ej + 2], (double)gjz);
+          atomicAdd(&sl.gs[3 * ek + 0], (double)gkx);
+          atomicAdd(&sl.gs[3 * ek + 1], (double)gky);
+          atomicAdd(&sl.gs[3 * ek + 2], (double)gkz);
+        }
       }
       wave_sync();
     }
 
     // ---- scatter of the adjoint to the forces ----
     if (ADJ) {
-      T* F = (T*)a.F;
-      T sx = 0, sy = 0, sz = 0;
+      double* F = a.F;
+      double sx = 0, sy = 0, sz = 0;
       for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
         if (e >= n) break;
-        const T gx = sl.gs[3 * e], gy = sl.gs[3 * e + 1], gz = sl.gs[3 * e + 2];
+        const double gx = sl.gs[3 * e], gy = sl.gs[3 * e + 1], gz = sl.gs[3 * e + 2];
         const long long jj = sl.j[e];
         unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 0], -gx);
         unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 1], -gy);
@@ -510,21 +576,13 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
   }
 }
 
-template <typename T, bool ADJ>
-size_t host_slice_bytes(int M, int nZ) {
-  using V = typename Vec4<T>::type;
-  size_t b = sizeof(V) * M + 2 * sizeof(int) * M + 2 * sizeof(T) * M;
-  if (ADJ) b += sizeof(T) * M + 3 * sizeof(T) * M;
-  b += sizeof(unsigned short) * (size_t)nZ * M + sizeof(int) * kMaxZ;
-  return (b + 31) & ~size_t(31);
-}
-
-template <typename T, bool ADJ>
-hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, hipStream_t s) {
+template <typename T, bool ADJ, int MQ>
+hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a1,
+                     hipStream_t s) {
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
   const int M = ((b->max_nb + 63) / 64) * 64 + 64;
-  const size_t sb = host_slice_bytes<T, ADJ>(M, m.nZ);
+  const size_t sb = slice_bytes<T, ADJ>(M, m.nZ);
   int wpb = 4;
   while (wpb > 1 && sb * wpb > 64 * 1024) wpb >>= 1;
   if (sb * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
@@ -546,20 +604,27 @@ hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, 
   a.a0 = a0;
   a.a1 = a1;
   a.maxnb = M;
-  a.G = b->G;
+  a.G = (double*)b->G;
   a.cG = b->cG;
-  a.F = b->F;
+  a.F = (double*)b->F;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
-  k_acsf<T, ADJ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+  k_acsf<T, ADJ, MQ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
   return hipGetLastError();
+}
+
+template <typename T, bool ADJ>
+hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, hipStream_t s) {
+  if (m.maxmem <= 4) return launch_mq<T, ADJ, 4>(b, m, a0, a1, s);
+  if (m.maxmem <= 8) return launch_mq<T, ADJ, 8>(b, m, a0, a1, s);
+  return launch_mq<T, ADJ, kMaxMembers>(b, m, a0, a1, s);
 }
 
 }  // namespace
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj) {
   const int M = ((max_nb + 63) / 64) * 64 + 64;
-  if (fp64) return adj ? host_slice_bytes<double, true>(M, nZ) : host_slice_bytes<double, false>(M, nZ);
-  return adj ? host_slice_bytes<float, true>(M, nZ) : host_slice_bytes<float, false>(M, nZ);
+  if (fp64) return adj ? slice_bytes<double, true>(M, nZ) : slice_bytes<double, false>(M, nZ);
+  return adj ? slice_bytes<float, true>(M, nZ) : slice_bytes<float, false>(M, nZ);
 }
 
 hipError_t launch_acsf_fwd(fnet_batch* b, const DevModel& m, long long a0, long long a1, int fp64,

@@ -244,7 +244,7 @@ int do_adjoint(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
   const int fp64 = b->precision == FNET_FP64;
-  const size_t el = fp64 ? sizeof(double) : sizeof(float);
+  const size_t el = sizeof(double);
   HIPCHECK(ctx, hipMemsetAsync(b->F, 0, el * (size_t)b->N * m.nout * 3, ctx->stream));
   if (b->launch == FNET_PER_STRUCT) {
     for (int st = 0; st < b->S; ++st)
@@ -428,6 +428,8 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   }
   m.nR = (int)rad.size();
   m.nAG = (int)grp.size();
+  m.maxmem = 1;
+  for (const AngGroup& a : grp) m.maxmem = std::max(m.maxmem, a.nmem);
   m.rcmax = rcmax;
   // network shape
   NetShape& sh = m.shape;
@@ -682,7 +684,7 @@ int fnet_acsf_forward(fnet_ctx* ctx, fnet_batch* b, double* G_out) {
   rc = do_acsf(b);
   if (rc) return rc;
   if (G_out)
-    return download_T(ctx, b->G, b->precision == FNET_FP64, (size_t)b->N * ctx->model.nG, G_out);
+    return download_T(ctx, b->G, 1, (size_t)b->N * ctx->model.nG, G_out);
   return FNET_OK;
 }
 
@@ -726,7 +728,7 @@ int fnet_forces(fnet_ctx* ctx, fnet_batch* b, double* F) {
     if (rc) return rc;
   }
   if (F)
-    return download_T(ctx, b->F, b->precision == FNET_FP64, (size_t)b->N * ctx->model.nout * 3, F);
+    return download_T(ctx, b->F, 1, (size_t)b->N * ctx->model.nout * 3, F);
   return FNET_OK;
 }
 

@@ -65,6 +65,7 @@ struct StructInfo {
 // Device copy of the model.
 struct DevModel {
   int nG = 0, nR = 0, nAG = 0, nZ = 0;
+  int maxmem = 1;               // largest angular group
   int zlist[kMaxZ];
   int nspecies = 0, nout = 0, nin = 0, next_in = 0;
   double rcmax = 0.0;
@@ -145,11 +146,11 @@ struct fnet_batch {
   int max_nb = 0;
   int* d_maxnb = nullptr;
   // results
-  void* G = nullptr;           // [N][nG] T
+  void* G = nullptr;           // [N][nG] FP64 (accumulated in FP64 in both precisions)
   void* cG = nullptr;          // [N][nout][nG] T
   double* Eatom = nullptr;     // [N][nout]
   double* Estruct = nullptr;   // [S][nout]
-  void* F = nullptr;           // [N][nout][3] T
+  void* F = nullptr;           // [N][nout][3] FP64
   double* lossgrad = nullptr;  // [N][nout]
   double* d_targets = nullptr; // staging of training inputs
   long long targets_cap = 0;

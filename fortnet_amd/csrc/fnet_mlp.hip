@@ -50,7 +50,7 @@ __device__ __forceinline__ T act_d(int id, T x) {
 struct MlpArgs {
   NetShape sh;
   const void* params; // [nspecies][nparams] T
-  const void* G;      // [N][nG] T
+  const double* G;    // [N][nG]
   const double* ext;
   int next;
   const int* ext_in;  // [next_in]
@@ -80,7 +80,7 @@ __device__ __forceinline__ void load_input(const MlpArgs& a, T* x, int cnt, int 
     if (at < cnt) {
       const long long atom = a.perm[start + at];
       if (f < a.nG) {
-        v = (T)(((double)((const T*)a.G)[atom * a.nG + f] - a.zmean[f]) * a.zinv[f]);
+        v = (T)((a.G[atom * a.nG + f] - a.zmean[f]) * a.zinv[f]);
       } else {
         v = (T)a.ext[atom * a.next + a.ext_in[f - a.nG]];
       }
@@ -359,7 +359,7 @@ MlpArgs make_args(fnet_batch* b, const DevModel& m, int first_tile) {
   MlpArgs a;
   a.sh = m.shape;
   a.params = std::is_same<T, double>::value ? (const void*)m.params : (const void*)m.params_f;
-  a.G = b->G;
+  a.G = (const double*)b->G;
   a.ext = b->ext;
   a.next = b->next;
   a.ext_in = m.ext_in;

@@ -13,7 +13,9 @@ through ctypes and adds the host-side glue of the reference's driver:
 * predictions / global sums / analytic forces, ``forces.F90:317-425``;
 * central finite-difference forces, ``forces.F90:182-313``.
 
-Parity status of each function is listed in ``oracle/README.md``.
+Parity unpinned: network activations other than tanh and sigmoid (no converted fixture uses
+them); everything else is pinned by the reference fixtures, closed forms, invariants or finite
+differences (tests/test_oracle.py).
 """
 import ctypes
 import os
