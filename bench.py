@@ -72,6 +72,40 @@ def cpu_baseline(geos, ns, budget_s=20.0):
                       "dG/dr, dense force contraction), %.1f s" % (done, atoms, dt)}
 
 
+def serving_bench(rates, duration_s, max_requests, serial_rates, serial_requests):
+    """Poisson request streams of ternary Si/C/O cells (16-4096 atoms, log-uniform, 48 features,
+    48-64-64-1 networks) through the continuous-batching loop and the one-at-a-time loop."""
+    from fortnet_amd import serve
+    geos, ns, desc = W.config("ternary", 0.5)
+    ctx = fa.Context(0)
+    ctx.upload_model(ns)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    ns = W.with_zscore(ns, b.acsf())
+    b.close()
+    ctx.close()
+    srv = serve.BatchingServer(ns, fa.FNET_FP32)
+    srv.run([serve.Request(k, g, 0.0) for k, g in enumerate(geos)])  # warm-up, buffer growth
+    out = {"workload": desc, "precision": "fp32", "mean_atoms_per_request":
+           float(np.mean([g.n for g in geos])), "rates": {}}
+    for rate in rates:
+        n = int(min(max_requests, max(50, rate * duration_s)))
+        arr = serve.poisson_arrivals(rate, n, 4000 + int(rate))
+        reqs = [serve.Request(k, geos[k % len(geos)], float(t)) for k, t in enumerate(arr)]
+        srv.batch_sizes = []
+        srv.run(reqs)
+        entry = {"batched": serve.latency_summary(reqs),
+                 "mean_requests_per_batch": float(np.mean(srv.batch_sizes))}
+        if rate in serial_rates:
+            sreqs = [serve.Request(k, geos[k % len(geos)], float(t))
+                     for k, t in enumerate(arr[:serial_requests])]
+            serve.serial_loop(ns, sreqs)
+            entry["serial"] = serve.latency_summary(sreqs)
+        out["rates"][str(int(rate))] = entry
+        print("serving rate %d: %s" % (rate, json.dumps(entry)), file=sys.stderr, flush=True)
+    srv.close()
+    return out
+
+
 def time_steps(batch, steps, warmup):
     for _ in range(warmup):
         batch.run()
@@ -97,6 +131,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="500,2000,5000,20000")
+    ap.add_argument("--serve-duration", type=float, default=1.0)
+    ap.add_argument("--serve-max-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -158,6 +196,12 @@ def main():
         "plain_stages_ms": dict(zip(names, [float(x) for x in stagep])),
         "roofline": roofline,
     }
+    b.close()
+    ctx.close()
+    if not args.no_serve:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serving"] = serving_bench(rates, args.serve_duration, args.serve_max_requests,
+                                       set(rates[:2]), 300)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(geos, ns, args.cpu_budget)
     print(json.dumps(out))

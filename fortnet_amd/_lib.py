@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libfnet_hip.so")
 
 FNET_FP64 = 0
 FNET_FP32 = 1
+FNET_FP32_NET64 = 2
 FNET_BATCHED = 0
 FNET_PER_STRUCT = 1
 
@@ -22,7 +23,7 @@ EXPORTS = [
     "fnet_batch_update_coords", "fnet_batch_set_mode", "fnet_batch_n_atoms",
     "fnet_neighbour_list", "fnet_get_neighbours", "fnet_acsf_forward", "fnet_energy",
     "fnet_forces", "fnet_run", "fnet_train_gradients", "fnet_optimizer_step_sd", "fnet_sync",
-    "fnet_stage_times",
+    "fnet_stage_times", "fnet_batch_reload", "fnet_query",
 ]
 
 
@@ -80,10 +81,12 @@ def load():
         "fnet_energy": (i32, [vp, vp, vp, vp]),
         "fnet_forces": (i32, [vp, vp, vp]),
         "fnet_run": (i32, [vp, vp]),
-        "fnet_train_gradients": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp]),
+        "fnet_train_gradients": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
         "fnet_optimizer_step_sd": (i32, [vp, dbl, dbl, dbl, vp, ctypes.POINTER(i32)]),
         "fnet_sync": (i32, [vp]),
         "fnet_stage_times": (i32, [vp, vp, i32]),
+        "fnet_batch_reload": (i32, [vp, ctypes.POINTER(BatchDesc)]),
+        "fnet_query": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -261,6 +261,16 @@ class Batch:
         self.ctx = ctx
         lib = ctx.lib
         self.lib = lib
+        d = self._describe(geos)
+        h = ctypes.c_void_p()
+        check(ctx.h, lib.fnet_batch_upload(ctx.h, ctypes.byref(d), ctypes.byref(h)),
+              "fnet_batch_upload")
+        self.h = h
+        ctx._batches.add(self)
+        self.set_mode(precision, launch)
+
+    def _describe(self, geos):
+        """Packs the geometries and returns the ``fnet_batch_desc`` pointing at them."""
         self.S = len(geos)
         n = [g.n for g in geos]
         self.offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(n)]), np.int64)
@@ -285,12 +295,19 @@ class Batch:
         d.atnum = ptr(self.atnum)
         d.n_ext = next_
         d.ext = ptr(self.ext)
-        h = ctypes.c_void_p()
-        check(ctx.h, lib.fnet_batch_upload(ctx.h, ctypes.byref(d), ctypes.byref(h)),
-              "fnet_batch_upload")
-        self.h = h
-        ctx._batches.add(self)
-        self.set_mode(precision, launch)
+        return d
+
+    def reload(self, geos):
+        """``fnet_batch_reload``: new structures in the same device buffers (serving)."""
+        d = self._describe(geos)
+        check(self.ctx.h, self.lib.fnet_batch_reload(self.h, ctypes.byref(d)), "fnet_batch_reload")
+
+    def query(self):
+        """True while work queued on the context is still running (``fnet_query``)."""
+        rc = self.lib.fnet_query(self.ctx.h)
+        if rc not in (0, 1):
+            check(self.ctx.h, rc, "fnet_query")
+        return rc == 1
 
     def close(self):
         if _shutdown[0]:
@@ -377,7 +394,10 @@ class Batch:
         dw = None if dp_weights is None else _f64(dp_weights).reshape(self.S)
         grad = np.zeros_like(m.params)
         pred = np.zeros((self.N, m.n_out))
+        loss = np.zeros(1)
         check(self.ctx.h, self.lib.fnet_train_gradients(self.ctx.h, self.h, ng, ptr(gt), ptr(at),
-                                                        ptr(aw), ptr(dw), ptr(grad), ptr(pred)),
+                                                        ptr(aw), ptr(dw), ptr(grad), ptr(pred),
+                                                        ptr(loss)),
               "fnet_train_gradients")
+        self.loss = float(loss[0])
         return grad, pred

@@ -88,20 +88,23 @@ struct AcsfArgs {
   long long a0, a1;
   int maxnb;
   double* G;      // [N][nG] (forward)
-  const void* cG; // [N][nout][nG] T (adjoint)
+  const double* cG; // [N][nout][nG] (adjoint)
   double* F;      // [N][nout][3] (adjoint)
 };
 
-// Bytes of one wave's LDS slice (host and device agree on this layout).
+// Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry: the
+// displacement and distance d, the pair-loop factors pa = (1/r, exp(-eta r^2) fc, atom-id
+// prefactor, radial derivative), the list-cutoff fc and fc', the neighbour index and element
+// code, the adjoint accumulator (adjoint only) and the per-element index lists.
 template <typename T, bool ADJ>
 __host__ __device__ size_t slice_bytes(int M, int nZ) {
-  size_t b = sizeof(typename Vec4<T>::type) * M;  // d
-  b += 5 * sizeof(T) * M;                          // ir, fc, dfc, E, q
-  if (ADJ) b += sizeof(T) * M + 3 * sizeof(double) * M;  // Ep, gs
-  b += 2 * sizeof(int) * M;                        // j, zc
-  b += 64 * sizeof(double);                        // radial partial sums
-  b += sizeof(int) * kMaxZ;                        // nz
-  b += sizeof(unsigned short) * (size_t)nZ * M;    // idx
+  size_t b = 2 * sizeof(typename Vec4<T>::type) * M;  // d, pa
+  b += 2 * sizeof(T) * M;                             // fc, dfc
+  if (ADJ) b += 3 * sizeof(T) * M;                    // gs
+  b += 2 * sizeof(int) * M;                           // j, zc
+  b += 64 * sizeof(double);                           // radial partial sums
+  b += sizeof(int) * kMaxZ;                           // nz
+  b += sizeof(unsigned short) * (size_t)nZ * M;       // idx
   return (b + 31) & ~size_t(31);
 }
 
@@ -109,8 +112,9 @@ template <typename T>
 struct Slice {
   using V = typename Vec4<T>::type;
   V* d;
-  T *ir, *fc, *dfc, *E, *q, *Ep;
-  double* gs;
+  V* pa;
+  T *fc, *dfc;
+  T* gs;
   int *j, *zc;
   double* red;
   int* nz;
@@ -123,30 +127,23 @@ __device__ Slice<T> make_slice(unsigned char* p, int M) {
   Slice<T> s;
   s.d = (V*)p;
   p += sizeof(V) * M;
-  s.ir = (T*)p;
-  p += sizeof(T) * M;
+  s.pa = (V*)p;
+  p += sizeof(V) * M;
+  s.red = (double*)p;
+  p += 64 * sizeof(double);
   s.fc = (T*)p;
   p += sizeof(T) * M;
   s.dfc = (T*)p;
   p += sizeof(T) * M;
-  s.E = (T*)p;
-  p += sizeof(T) * M;
-  s.q = (T*)p;
-  p += sizeof(T) * M;
-  s.Ep = nullptr;
   s.gs = nullptr;
   if (ADJ) {
-    s.Ep = (T*)p;
-    p += sizeof(T) * M;
-    s.gs = (double*)p;
-    p += 3 * sizeof(double) * M;
+    s.gs = (T*)p;
+    p += 3 * sizeof(T) * M;
   }
   s.j = (int*)p;
   p += sizeof(int) * M;
   s.zc = (int*)p;
   p += sizeof(int) * M;
-  s.red = (double*)p;
-  p += 64 * sizeof(double);
   s.nz = (int*)p;
   p += sizeof(int) * kMaxZ;
   s.idx = (unsigned short*)p;
@@ -218,7 +215,7 @@ struct PairIter {
 };
 
 template <typename T, bool ADJ, int MQ>
-__global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
+__global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
   const int lane = threadIdx.x & 63;
@@ -248,7 +245,6 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
       sl.j[e] = jj;
       zc = a.zcode[jj];
       sl.zc[e] = zc;
-      sl.ir[e] = (T)1 / d.w;
       const T x = pi * d.w / rcl;
       sl.fc[e] = (T)0.5 * (fcos(x) + (T)1);
       sl.dfc[e] = -(T)0.5 * (pi / rcl) * fsin(x);
@@ -269,7 +265,7 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
 
   const int nouts = ADJ ? a.nout : 1;
   for (int t = 0; t < nouts; ++t) {
-    const T* cG = ADJ ? (const T*)a.cG + (i * a.nout + t) * a.nG : nullptr;
+    const double* cG = ADJ ? a.cG + (i * a.nout + t) * a.nG : nullptr;
 
     // ---- radial functions ----
     if (!ADJ) {
@@ -322,7 +318,7 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
         T s = 0;
         for (int f = 0; f < a.nR; ++f) {
           const RadialFn& fn = a.radial[f];
-          const T cf = cG[fn.col];
+          const T cf = (T)cG[fn.col];
           if (cf == (T)0) continue;
           if (!(fn.zc < 0 || fn.zc == zc) || !(r <= (T)fn.rc)) continue;
           T fc, dfc;
@@ -349,10 +345,10 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
           }
           s += cf * g;
         }
-        const double sr = (double)(s * sl.ir[e]);
-        sl.gs[3 * e + 0] = sr * (double)d.x;
-        sl.gs[3 * e + 1] = sr * (double)d.y;
-        sl.gs[3 * e + 2] = sr * (double)d.z;
+        const T sr = s / r;
+        sl.gs[3 * e + 0] = sr * d.x;
+        sl.gs[3 * e + 1] = sr * d.y;
+        sl.gs[3 * e + 2] = sr * d.z;
       }
       wave_sync();
     }
@@ -366,7 +362,7 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
         bool any = false;
 #pragma unroll
         for (int q = 0; q < MQ; ++q) {
-          A[q] = (q < nm) ? cG[grp.col[q]] * (T)grp.scale[q] : (T)0;
+          A[q] = (q < nm) ? (T)(cG[grp.col[q]] * grp.scale[q]) : (T)0;
           any = any || (A[q] != (T)0);
         }
         if (!any) continue;
@@ -401,9 +397,12 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
           E = ex * fc;
           Ep = ex * (dfc - (T)2 * eta * r * fc);
         }
-        sl.q[e] = qj;
-        sl.E[e] = E;
-        if (ADJ) sl.Ep[e] = Ep;
+        V pa;
+        pa.x = (T)1 / r;
+        pa.y = E;
+        pa.z = qj;
+        pa.w = Ep;
+        sl.pa[e] = pa;
       }
       wave_sync();
 
@@ -427,15 +426,16 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
         while (it.next(ej, ek, diag, wi)) {
           const T w = (T)wi;
           const V dj = sl.d[ej], dk = sl.d[ek];
-          T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * sl.ir[ej] * sl.ir[ek];
+          const V pj = sl.pa[ej], pk = sl.pa[ek];
+          T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
           c = fmin(fmax(c, (T)-1), (T)1);
-          T base = w * sl.E[ej] * sl.E[ek];
+          T base = w * pj.y * pk.y;
           if (g4) {
             const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
             const T r2 = rx * rx + ry * ry + rz * rz;
             const T rjk = sqrt(r2);
             if (rjk <= rc) {
-              const T fcjk = (T)0.5 * sl.q[ej] * sl.q[ek] * (fcos(pi * rjk / rc) + (T)1);
+              const T fcjk = (T)0.5 * pj.z * pk.z * (fcos(pi * rjk / rc) + (T)1);
               base *= fexp(-eta * r2) * fcjk;
             } else {
               base = 0;
@@ -465,19 +465,21 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
         while (it.next(ej, ek, diag, wi)) {
           const T w = (T)wi;
           const V dj = sl.d[ej];
-          const T irj = sl.ir[ej];
-          const T Ej = sl.E[ej], Epj = sl.Ep[ej];
+          const V pj = sl.pa[ej];
+          const T irj = pj.x;
+          const T Ej = pj.y, Epj = pj.w;
           if (diag) {
-            const T H = g4 ? sl.q[ej] * sl.q[ej] : (T)1;
+            const T H = g4 ? pj.z * pj.z : (T)1;
             const T gsc = S0d * (T)2 * Ej * Epj * H * irj;
-            atomicAdd(&sl.gs[3 * ej + 0], (double)(gsc * dj.x));
-            atomicAdd(&sl.gs[3 * ej + 1], (double)(gsc * dj.y));
-            atomicAdd(&sl.gs[3 * ej + 2], (double)(gsc * dj.z));
+            atomicAdd(&sl.gs[3 * ej + 0], gsc * dj.x);
+            atomicAdd(&sl.gs[3 * ej + 1], gsc * dj.y);
+            atomicAdd(&sl.gs[3 * ej + 2], gsc * dj.z);
             continue;
           }
           const V dk = sl.d[ek];
-          const T irk = sl.ir[ek];
-          const T Ek = sl.E[ek], Epk = sl.Ep[ek];
+          const V pk = sl.pa[ek];
+          const T irk = pk.x;
+          const T Ek = pk.y, Epk = pk.w;
           T H = 1, Hp = 0, rjk = 1;
           T rx = 0, ry = 0, rz = 0;
           if (g4) {
@@ -487,7 +489,7 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
             const T r2 = rx * rx + ry * ry + rz * rz;
             rjk = sqrt(r2);
             if (!(rjk <= rc)) continue;
-            const T qq = sl.q[ej] * sl.q[ek];
+            const T qq = pj.z * pk.z;
             const T x = pi * rjk / rc;
             const T ex = fexp(-eta * r2);
             const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
@@ -536,12 +538,12 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
             gky -= h * ry;
             gkz -= h * rz;
           }
-          atomicAdd(&sl.gs[3 * ej + 0], (double)gjx);
-          atomicAdd(&sl.gs[3 * ej + 1], (double)gjy);
-          atomicAdd(&sl.gs[3 * ej + 2], (double)gjz);
-          atomicAdd(&sl.gs[3 * ek + 0], (double)gkx);
-          atomicAdd(&sl.gs[3 * ek + 1], (double)gky);
-          atomicAdd(&sl.gs[3 * ek + 2], (double)gkz);
+          atomicAdd(&sl.gs[3 * ej + 0], gjx);
+          atomicAdd(&sl.gs[3 * ej + 1], gjy);
+          atomicAdd(&sl.gs[3 * ej + 2], gjz);
+          atomicAdd(&sl.gs[3 * ek + 0], gkx);
+          atomicAdd(&sl.gs[3 * ek + 1], gky);
+          atomicAdd(&sl.gs[3 * ek + 2], gkz);
         }
       }
       wave_sync();
@@ -554,7 +556,8 @@ __global__ void __launch_bounds__(256) k_acsf(AcsfArgs a) {
       for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
         if (e >= n) break;
-        const double gx = sl.gs[3 * e], gy = sl.gs[3 * e + 1], gz = sl.gs[3 * e + 2];
+        const double gx = (double)sl.gs[3 * e], gy = (double)sl.gs[3 * e + 1],
+                     gz = (double)sl.gs[3 * e + 2];
         const long long jj = sl.j[e];
         unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 0], -gx);
         unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 1], -gy);
@@ -581,9 +584,9 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
                      hipStream_t s) {
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
-  const int M = ((b->max_nb + 63) / 64) * 64 + 64;
+  const int M = ((b->max_nb + 31) / 32) * 32 + 32;
   const size_t sb = slice_bytes<T, ADJ>(M, m.nZ);
-  int wpb = 4;
+  int wpb = 8;
   while (wpb > 1 && sb * wpb > 64 * 1024) wpb >>= 1;
   if (sb * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
   AcsfArgs a;
@@ -605,7 +608,7 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.a1 = a1;
   a.maxnb = M;
   a.G = (double*)b->G;
-  a.cG = b->cG;
+  a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
   k_acsf<T, ADJ, MQ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
@@ -622,7 +625,7 @@ hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, 
 }  // namespace
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj) {
-  const int M = ((max_nb + 63) / 64) * 64 + 64;
+  const int M = ((max_nb + 31) / 32) * 32 + 32;
   if (fp64) return adj ? slice_bytes<double, true>(M, nZ) : slice_bytes<double, false>(M, nZ);
   return adj ? slice_bytes<float, true>(M, nZ) : slice_bytes<float, false>(M, nZ);
 }

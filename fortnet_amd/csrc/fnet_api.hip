@@ -80,7 +80,7 @@ void invalidate(fnet_batch* b) {
 int ensure_tiles(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
-  const int TA = mlp_tile_atoms(m.shape, b->precision == FNET_FP64);
+  const int TA = mlp_tile_atoms(m.shape, b->precision != FNET_FP32);
   if (b->tile_atoms == TA && b->tile_launch == b->launch && b->tile_sp) return FNET_OK;
   std::vector<int> perm, tsp, tstart, tcnt;
   perm.reserve(b->N);
@@ -130,13 +130,21 @@ int ensure_tiles(fnet_batch* b) {
 int ensure_results(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
-  if (b->G) return FNET_OK;
-  HIPCHECK(ctx, dalloc((double**)&b->G, (size_t)b->N * m.nG));
-  HIPCHECK(ctx, dalloc((double**)&b->cG, (size_t)b->N * m.nout * m.nG));
-  HIPCHECK(ctx, dalloc(&b->Eatom, (size_t)b->N * m.nout));
-  HIPCHECK(ctx, dalloc(&b->Estruct, (size_t)b->S * m.nout));
-  HIPCHECK(ctx, dalloc((double**)&b->F, (size_t)b->N * m.nout * 3));
-  HIPCHECK(ctx, dalloc(&b->lossgrad, (size_t)b->N * m.nout));
+  if (b->G && b->res_N >= b->N && b->res_S >= b->S && b->res_nG == m.nG && b->res_nout == m.nout)
+    return FNET_OK;
+  void* pr[] = {b->G, b->cG, b->Eatom, b->Estruct, b->F, b->lossgrad};
+  for (void* p : pr) dfree(p);
+  const long long N = std::max<long long>(b->N, b->capN), S = std::max<long long>(b->S, b->capS);
+  HIPCHECK(ctx, dalloc((double**)&b->G, (size_t)N * m.nG));
+  HIPCHECK(ctx, dalloc((double**)&b->cG, (size_t)N * m.nout * m.nG));
+  HIPCHECK(ctx, dalloc(&b->Eatom, (size_t)N * m.nout));
+  HIPCHECK(ctx, dalloc(&b->Estruct, (size_t)S * m.nout));
+  HIPCHECK(ctx, dalloc((double**)&b->F, (size_t)N * m.nout * 3));
+  HIPCHECK(ctx, dalloc(&b->lossgrad, (size_t)N * m.nout));
+  b->res_N = N;
+  b->res_S = S;
+  b->res_nG = m.nG;
+  b->res_nout = m.nout;
   return FNET_OK;
 }
 
@@ -150,12 +158,16 @@ int do_neighbours(fnet_batch* b) {
   hipStream_t s = ctx->stream;
   const double rc = m.rcmax;
   HIPCHECK(ctx, launch_struct_prep(b, rc, s));
+  HIPCHECK(ctx, launch_cell_bin(b, s));
   HIPCHECK(ctx, hipMemsetAsync(b->d_maxnb, 0, sizeof(int), s));
   if (b->launch == FNET_PER_STRUCT) {
-    for (int st = 0; st < b->S; ++st)
+    for (int st = 0; st < b->S; ++st) {
       HIPCHECK(ctx, launch_neigh_count(b, rc, b->h_off[st], b->h_off[st + 1], s));
+      HIPCHECK(ctx, launch_cells_count(b, rc, b->h_off[st], b->h_off[st + 1], s));
+    }
   } else {
     HIPCHECK(ctx, launch_neigh_count(b, rc, 0, b->N, s));
+    HIPCHECK(ctx, launch_cells_count(b, rc, 0, b->N, s));
   }
   HIPCHECK(ctx, launch_scan(b->cnt, b->rowptr, b->N, b->scan_tmp, s));
   int hv[2] = {0, 0};
@@ -186,10 +198,13 @@ int do_neighbours(fnet_batch* b) {
   b->npairs = P;
   const int fp64 = b->precision == FNET_FP64;
   if (b->launch == FNET_PER_STRUCT) {
-    for (int st = 0; st < b->S; ++st)
+    for (int st = 0; st < b->S; ++st) {
       HIPCHECK(ctx, launch_neigh_fill(b, rc, b->h_off[st], b->h_off[st + 1], fp64, s));
+      HIPCHECK(ctx, launch_cells_fill(b, rc, b->h_off[st], b->h_off[st + 1], fp64, s));
+    }
   } else {
     HIPCHECK(ctx, launch_neigh_fill(b, rc, 0, b->N, fp64, s));
+    HIPCHECK(ctx, launch_cells_fill(b, rc, 0, b->N, fp64, s));
   }
   b->have_nl = true;
   b->result_precision = b->precision;
@@ -213,7 +228,7 @@ int do_acsf(fnet_batch* b) {
 int do_mlp(fnet_batch* b, int gradin) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
-  const int fp64 = b->precision == FNET_FP64;
+  const int fp64 = b->precision != FNET_FP32;
   int rc = ensure_tiles(b);
   if (rc) return rc;
   if (b->launch == FNET_PER_STRUCT) {
@@ -253,6 +268,75 @@ int do_adjoint(fnet_batch* b) {
     HIPCHECK(ctx, launch_acsf_adj(b, m, 0, b->N, fp64, ctx->stream));
   }
   b->have_F = true;
+  return FNET_OK;
+}
+
+// Cell-list set-up of the batch from the lattices: a periodic structure of at least
+// kCellMinAtoms atoms whose cell allows >= 3 bins of perpendicular width >= rc on every axis is
+// searched with cell lists, any other with the direct candidate scan.
+int setup_cells(fnet_batch* b, const double* lat, const int* per) {
+  fnet_ctx* ctx = b->ctx;
+  const double rc = ctx->model.rcmax;
+  std::vector<int> info(4 * (size_t)b->S, 0), coff(b->S + 1, 0);
+  long long total = 0;
+  for (int s = 0; s < b->S; ++s) {
+    coff[s] = (int)total;
+    const long long n = b->h_off[s + 1] - b->h_off[s];
+    if (!per[s] || n < kCellMinAtoms) continue;
+    const double* L = lat + 9 * s;
+    auto o = [&](int r, int c) { return L[c * 3 + r]; };
+    const double det = o(0, 0) * (o(1, 1) * o(2, 2) - o(2, 1) * o(1, 2)) -
+                       o(0, 1) * (o(1, 0) * o(2, 2) - o(2, 0) * o(1, 2)) +
+                       o(0, 2) * (o(1, 0) * o(2, 1) - o(2, 0) * o(1, 1));
+    int nb[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      // row a of inv(L) = (cofactor column a) / det
+      const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+      double ss = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const double cof = o(c1, a1) * o(c2, a2) - o(c1, a2) * o(c2, a1);
+        ss += (cof / det) * (cof / det);
+      }
+      const double w = rc * std::sqrt(ss);
+      const double nbf = std::floor(1.0 / (w * (1.0 + 1e-9) + 1e-12));
+      nb[a] = nbf > 1000.0 ? 1000 : (int)nbf;
+      ok = ok && nb[a] >= 3;
+    }
+    if (!ok) continue;
+    info[4 * s + 0] = nb[0];
+    info[4 * s + 1] = nb[1];
+    info[4 * s + 2] = nb[2];
+    info[4 * s + 3] = 1;
+    total += (long long)nb[0] * nb[1] * nb[2];
+  }
+  coff[b->S] = (int)total;
+  if (total >= (1LL << 30)) return fail(ctx, FNET_ERR_CAPACITY, "too many cells");
+  hipStream_t s = ctx->stream;
+  if (!b->cellinfo) {
+    HIPCHECK(ctx, dalloc(&b->cellinfo, (size_t)b->S));
+    HIPCHECK(ctx, dalloc(&b->cell_off, (size_t)b->S + 1));
+    HIPCHECK(ctx, dalloc(&b->atom_cell, (size_t)b->N));
+    HIPCHECK(ctx, dalloc(&b->atom_m, (size_t)b->N));
+    HIPCHECK(ctx, dalloc(&b->cell_atoms, (size_t)b->N));
+  }
+  if (total != b->ncells) {
+    dfree(b->cell_count);
+    dfree(b->cell_start);
+    dfree(b->cell_scan_tmp);
+    b->cell_count = b->cell_start = b->cell_scan_tmp = nullptr;
+    HIPCHECK(ctx, dalloc(&b->cell_count, (size_t)total));
+    HIPCHECK(ctx, dalloc(&b->cell_start, (size_t)total + 1));
+    HIPCHECK(ctx, dalloc(&b->cell_scan_tmp, (size_t)scan_tmp_size(total)));
+    b->ncells = total;
+  }
+  b->h_cellinfo = info;
+  HIPCHECK(ctx, hipMemcpyAsync(b->cellinfo, info.data(), sizeof(int) * info.size(),
+                               hipMemcpyHostToDevice, s));
+  HIPCHECK(ctx, hipMemcpyAsync(b->cell_off, coff.data(), sizeof(int) * coff.size(),
+                               hipMemcpyHostToDevice, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
   return FNET_OK;
 }
 
@@ -430,6 +514,11 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   m.nAG = (int)grp.size();
   m.maxmem = 1;
   for (const AngGroup& a : grp) m.maxmem = std::max(m.maxmem, a.nmem);
+  m.need_ext = 0;
+  for (const RadialFn& f : rad) m.need_ext = std::max(m.need_ext, f.atomid);
+  for (const AngGroup& a : grp) m.need_ext = std::max(m.need_ext, a.atomid);
+  for (int k = 0; k < md->n_ext_in; ++k)
+    if (md->ext_in_idx) m.need_ext = std::max(m.need_ext, md->ext_in_idx[k]);
   m.rcmax = rcmax;
   // network shape
   NetShape& sh = m.shape;
@@ -493,10 +582,14 @@ int fnet_params_upload(fnet_ctx* ctx, const double* params) {
   return FNET_OK;
 }
 
-int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* d, fnet_batch** out) {
-  if (!ctx) return FNET_ERR_ARG;
-  if (!d || !out) return fail(ctx, FNET_ERR_ARG, "NULL argument");
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Validates a batch descriptor and loads it into b, reusing b's device buffers when they are
+// large enough (fnet_batch_reload) or allocating them (fnet_batch_upload).
+int load_batch(fnet_ctx* ctx, fnet_batch* b, const fnet_batch_desc* d) {
+  if (!d) return fail(ctx, FNET_ERR_ARG, "NULL batch descriptor");
   if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "upload the model before a batch");
   if (d->n_struct < 0 || !d->offsets) return fail(ctx, FNET_ERR_ARG, "invalid structure count");
   const long long N = d->n_struct ? d->offsets[d->n_struct] : 0;
@@ -508,23 +601,16 @@ int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* d, fnet_batch** out)
     return fail(ctx, FNET_ERR_ARG, "coords/species/atnum missing");
   if (d->n_ext < 0 || (d->n_ext > 0 && !d->ext)) return fail(ctx, FNET_ERR_ARG, "invalid ext");
   const DevModel& m = ctx->model;
+  if (m.need_ext > d->n_ext)
+    return fail(ctx, FNET_ERR_ARG, "the model needs more external feature columns than given");
   HIPCHECK(ctx, hipSetDevice(ctx->device));
-  fnet_batch* b = new fnet_batch();
-  b->ctx = ctx;
-  b->S = d->n_struct;
-  b->N = N;
-  b->next = d->n_ext;
-  b->h_off.assign(d->offsets, d->offsets + d->n_struct + 1);
-  b->h_species.assign(d->species, d->species + N);
-  std::vector<int> per(d->n_struct, 0), astruct(N), zc(N);
-  std::vector<double> lat((size_t)9 * d->n_struct, 0.0);
-  for (int s = 0; s < d->n_struct; ++s) {
+  const int S = d->n_struct;
+  std::vector<int> per(S, 0), astruct(N), zc(N);
+  std::vector<double> lat((size_t)9 * S, 0.0);
+  for (int s = 0; s < S; ++s) {
     per[s] = d->periodic ? (d->periodic[s] != 0) : 0;
     if (per[s]) {
-      if (!d->latvecs) {
-        delete b;
-        return fail(ctx, FNET_ERR_ARG, "periodic structure without lattice vectors");
-      }
+      if (!d->latvecs) return fail(ctx, FNET_ERR_ARG, "periodic structure without lattice vectors");
       for (int q = 0; q < 9; ++q) lat[9 * s + q] = d->latvecs[9 * s + q];
       // fnetdata.F90:1151-1154 rejects dependent lattice vectors; the image ranges of
       // latpointiter.F90:226-253 must also stay finite and modest
@@ -533,11 +619,9 @@ int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* d, fnet_batch** out)
       const double det = o(0, 0) * (o(1, 1) * o(2, 2) - o(2, 1) * o(1, 2)) -
                          o(0, 1) * (o(1, 0) * o(2, 2) - o(2, 0) * o(1, 2)) +
                          o(0, 2) * (o(1, 0) * o(2, 1) - o(2, 0) * o(1, 1));
-      if (!(std::fabs(det) >= 1e-12)) {
-        delete b;
+      if (!(std::fabs(det) >= 1e-12))
         return fail(ctx, FNET_ERR_ARG, "dependent lattice vectors in structure " +
                                            std::to_string(s));
-      }
       double cof[3][3];
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) {
@@ -551,56 +635,120 @@ int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* d, fnet_batch** out)
         for (int c = 0; c < 3; ++c) ss += (cof[c][a] / det) * (cof[c][a] / det);
         nimg *= 2.0 * (std::floor(m.rcmax * std::sqrt(ss)) + 1.0) + 1.0;
       }
-      if (!(nimg <= 1.0e6)) {
-        delete b;
+      if (!(nimg <= 1.0e6))
         return fail(ctx, FNET_ERR_CAPACITY, "cell too small for the cutoff (more than 1e6 "
                                             "lattice images) in structure " + std::to_string(s));
-      }
     }
     for (long long i = d->offsets[s]; i < d->offsets[s + 1]; ++i) astruct[i] = s;
   }
   for (long long i = 0; i < N; ++i) {
-    if (d->species[i] < 0 || d->species[i] >= m.nspecies) {
-      delete b;
+    if (d->species[i] < 0 || d->species[i] >= m.nspecies)
       return fail(ctx, FNET_ERR_ARG, "species index out of range");
-    }
     zc[i] = zcode_of(m, d->atnum[i]);
   }
-  // every ext column used by the model must exist
-  int need_ext = 0;
-  std::vector<int> ext_in(m.next_in);
-  if (m.next_in > 0)
-    (void)hipMemcpy(ext_in.data(), m.ext_in, sizeof(int) * m.next_in, hipMemcpyDeviceToHost);
-  for (int k = 0; k < m.next_in; ++k) need_ext = std::max(need_ext, ext_in[k] + 1);
-  {
-    std::vector<RadialFn> rad(m.nR);
-    std::vector<AngGroup> grp(m.nAG);
-    if (m.nR) (void)hipMemcpy(rad.data(), m.radial, sizeof(RadialFn) * m.nR, hipMemcpyDeviceToHost);
-    if (m.nAG) (void)hipMemcpy(grp.data(), m.groups, sizeof(AngGroup) * m.nAG, hipMemcpyDeviceToHost);
-    for (auto& f : rad) need_ext = std::max(need_ext, f.atomid);
-    for (auto& g : grp) need_ext = std::max(need_ext, g.atomid);
-  }
-  if (need_ext > d->n_ext) {
-    delete b;
-    return fail(ctx, FNET_ERR_ARG, "the model needs more external feature columns than given");
-  }
   hipStream_t s = ctx->stream;
-  HIPCHECK(ctx, upload(&b->xyz, d->coords, (size_t)3 * N, s));
-  HIPCHECK(ctx, upload(&b->lat, lat.data(), lat.size(), s));
-  HIPCHECK(ctx, upload(&b->periodic, per.data(), per.size(), s));
-  HIPCHECK(ctx, upload(&b->off, d->offsets, (size_t)d->n_struct + 1, s));
-  HIPCHECK(ctx, upload(&b->atom_struct, astruct.data(), astruct.size(), s));
-  HIPCHECK(ctx, upload(&b->species, d->species, (size_t)N, s));
-  HIPCHECK(ctx, upload(&b->zcode, zc.data(), zc.size(), s));
-  HIPCHECK(ctx, upload(&b->ext, d->ext, (size_t)N * d->n_ext, s));
-  HIPCHECK(ctx, dalloc(&b->sinfo, (size_t)d->n_struct));
-  HIPCHECK(ctx, dalloc(&b->cnt, (size_t)N));
-  HIPCHECK(ctx, dalloc(&b->rowptr, (size_t)N + 1));
-  HIPCHECK(ctx, dalloc(&b->scan_tmp, (size_t)scan_tmp_size(N)));
-  HIPCHECK(ctx, dalloc(&b->d_maxnb, 1));
   HIPCHECK(ctx, hipStreamSynchronize(s));
+  // capacity: atoms
+  if (N > b->capN || !b->xyz) {
+    void* pa[] = {b->xyz, b->atom_struct, b->species, b->zcode, b->cnt, b->rowptr, b->scan_tmp,
+                  b->atom_cell, b->atom_m, b->cell_atoms};
+    for (void* p : pa) dfree(p);
+    const long long cap = std::max<long long>(N, 1);
+    HIPCHECK(ctx, dalloc(&b->xyz, (size_t)3 * cap));
+    HIPCHECK(ctx, dalloc(&b->atom_struct, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->species, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->zcode, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->cnt, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->rowptr, (size_t)cap + 1));
+    HIPCHECK(ctx, dalloc(&b->scan_tmp, (size_t)scan_tmp_size(cap)));
+    HIPCHECK(ctx, dalloc(&b->atom_cell, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->atom_m, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->cell_atoms, (size_t)cap));
+    b->capN = cap;
+  }
+  if (N * d->n_ext > b->capExt || !b->ext) {
+    dfree(b->ext);
+    b->capExt = std::max<long long>(N * d->n_ext, 1);
+    HIPCHECK(ctx, dalloc(&b->ext, (size_t)b->capExt));
+  }
+  // capacity: structures
+  if (S > b->capS || !b->lat) {
+    void* ps[] = {b->lat, b->periodic, b->off, b->sinfo, b->cellinfo, b->cell_off};
+    for (void* p : ps) dfree(p);
+    const long long cap = std::max(S, 1);
+    HIPCHECK(ctx, dalloc(&b->lat, (size_t)9 * cap));
+    HIPCHECK(ctx, dalloc(&b->periodic, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->off, (size_t)cap + 1));
+    HIPCHECK(ctx, dalloc(&b->sinfo, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->cellinfo, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->cell_off, (size_t)cap + 1));
+    b->capS = cap;
+  }
+  if (!b->d_maxnb) HIPCHECK(ctx, dalloc(&b->d_maxnb, 1));
+  b->S = S;
+  b->N = N;
+  b->next = d->n_ext;
+  b->h_off.assign(d->offsets, d->offsets + S + 1);
+  b->h_species.assign(d->species, d->species + N);
+  b->h_lat = lat;
+  b->h_periodic = per;
+  if (N) HIPCHECK(ctx, hipMemcpyAsync(b->xyz, d->coords, sizeof(double) * 3 * N,
+                                      hipMemcpyHostToDevice, s));
+  if (S) {
+    HIPCHECK(ctx, hipMemcpyAsync(b->lat, lat.data(), sizeof(double) * lat.size(),
+                                 hipMemcpyHostToDevice, s));
+    HIPCHECK(ctx, hipMemcpyAsync(b->periodic, per.data(), sizeof(int) * S, hipMemcpyHostToDevice,
+                                 s));
+  }
+  HIPCHECK(ctx, hipMemcpyAsync(b->off, d->offsets, sizeof(long long) * (S + 1),
+                               hipMemcpyHostToDevice, s));
+  if (N) {
+    HIPCHECK(ctx, hipMemcpyAsync(b->atom_struct, astruct.data(), sizeof(int) * N,
+                                 hipMemcpyHostToDevice, s));
+    HIPCHECK(ctx, hipMemcpyAsync(b->species, d->species, sizeof(int) * N, hipMemcpyHostToDevice,
+                                 s));
+    HIPCHECK(ctx, hipMemcpyAsync(b->zcode, zc.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+  }
+  if (N * d->n_ext)
+    HIPCHECK(ctx, hipMemcpyAsync(b->ext, d->ext, sizeof(double) * N * d->n_ext,
+                                 hipMemcpyHostToDevice, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  invalidate(b);
+  b->tile_atoms = 0;  // species layout changed: rebuild the network tiles
+  b->result_precision = -1;
+  return setup_cells(b, lat.data(), per.data());
+}
+
+}  // namespace
+
+extern "C" {
+
+int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* d, fnet_batch** out) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!d || !out) return fail(ctx, FNET_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  fnet_batch* b = new fnet_batch();
+  b->ctx = ctx;
+  const int rc = load_batch(ctx, b, d);
+  if (rc) {
+    fnet_batch_destroy(b);
+    return rc;
+  }
   *out = b;
   return FNET_OK;
+}
+
+int fnet_batch_reload(fnet_batch* b, const fnet_batch_desc* d) {
+  if (!b) return FNET_ERR_ARG;
+  return load_batch(b->ctx, b, d);
+}
+
+int fnet_query(fnet_ctx* ctx) {
+  if (!ctx) return FNET_ERR_ARG;
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorNotReady) return 1;
+  return hipfail(ctx, e, "hipStreamQuery");
 }
 
 void fnet_batch_destroy(fnet_batch* b) {
@@ -610,7 +758,9 @@ void fnet_batch_destroy(fnet_batch* b) {
   void* ptrs[] = {b->xyz, b->lat, b->periodic, b->off, b->atom_struct, b->species, b->zcode,
                   b->ext, b->sinfo, b->perm, b->tile_sp, b->tile_start, b->tile_cnt, b->cnt,
                   b->rowptr, b->scan_tmp, b->nj, b->nimg, b->nd, b->d_maxnb, b->G, b->cG,
-                  b->Eatom, b->Estruct, b->F, b->lossgrad, b->d_targets};
+                  b->Eatom, b->Estruct, b->F, b->lossgrad, b->d_targets, b->cellinfo,
+                  b->cell_off, b->cell_count, b->cell_start, b->cell_atoms, b->atom_cell,
+                  b->atom_m, b->cell_scan_tmp};
   for (void* p : ptrs) dfree(p);
   if (b->ctx->last_train_batch == b) b->ctx->last_train_batch = nullptr;
   delete b;
@@ -623,9 +773,14 @@ int fnet_batch_update_coords(fnet_batch* b, const double* coords, const double* 
   HIPCHECK(ctx, hipSetDevice(ctx->device));
   HIPCHECK(ctx, hipMemcpyAsync(b->xyz, coords, sizeof(double) * 3 * b->N, hipMemcpyHostToDevice,
                                ctx->stream));
-  if (latvecs)
+  if (latvecs) {
     HIPCHECK(ctx, hipMemcpyAsync(b->lat, latvecs, sizeof(double) * 9 * b->S,
                                  hipMemcpyHostToDevice, ctx->stream));
+    for (int q = 0; q < 9 * b->S; ++q)
+      if (b->h_periodic[q / 9]) b->h_lat[q] = latvecs[q];
+    const int crc = setup_cells(b, b->h_lat.data(), b->h_periodic.data());
+    if (crc) return crc;
+  }
   HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
   invalidate(b);
   return FNET_OK;
@@ -633,7 +788,7 @@ int fnet_batch_update_coords(fnet_batch* b, const double* coords, const double* 
 
 int fnet_batch_set_mode(fnet_batch* b, int precision, int launch) {
   if (!b) return FNET_ERR_ARG;
-  if (precision != FNET_FP64 && precision != FNET_FP32)
+  if (precision != FNET_FP64 && precision != FNET_FP32 && precision != FNET_FP32_NET64)
     return fail(b->ctx, FNET_ERR_ARG, "unknown precision");
   if (launch != FNET_BATCHED && launch != FNET_PER_STRUCT)
     return fail(b->ctx, FNET_ERR_ARG, "unknown launch mode");
@@ -772,7 +927,8 @@ int fnet_stage_times(fnet_ctx* ctx, double* ms, int n) {
 
 int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const double* g_targets,
                          const double* a_targets, const double* atomic_weights,
-                         const double* dp_weights, double* grad, double* pred) {
+                         const double* dp_weights, double* grad, double* pred,
+                         double* loss) {
   int rc = check_ready(ctx, b);
   if (rc) return rc;
   const DevModel& m = ctx->model;
@@ -828,12 +984,17 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
     ctx->grad_cap = np;
   }
   HIPCHECK(ctx, hipMemsetAsync(ctx->d_grad, 0, sizeof(double) * np, s));
-  HIPCHECK(ctx, launch_mlp_grad(b, m, b->precision == FNET_FP64, ctx->d_grad, s));
+  HIPCHECK(ctx, launch_mlp_grad(b, m, b->precision != FNET_FP32, ctx->d_grad, s));
+  double* dloss = dpw + b->S;
+  HIPCHECK(ctx, launch_msloss(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
+                              dp_weights ? dpw : nullptr, dloss, s));
   if (grad)
     HIPCHECK(ctx, hipMemcpyAsync(grad, ctx->d_grad, sizeof(double) * np, hipMemcpyDeviceToHost, s));
   if (pred)
     HIPCHECK(ctx, hipMemcpyAsync(pred, b->Eatom, sizeof(double) * b->N * m.nout,
                                  hipMemcpyDeviceToHost, s));
+  if (loss)
+    HIPCHECK(ctx, hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHECK(ctx, hipStreamSynchronize(s));
   ctx->have_grad = true;
   ctx->last_train_batch = b;

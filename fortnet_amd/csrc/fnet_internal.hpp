@@ -66,6 +66,7 @@ struct StructInfo {
 struct DevModel {
   int nG = 0, nR = 0, nAG = 0, nZ = 0;
   int maxmem = 1;               // largest angular group
+  int need_ext = 0;             // external feature columns the model reads
   int zlist[kMaxZ];
   int nspecies = 0, nout = 0, nin = 0, next_in = 0;
   double rcmax = 0.0;
@@ -112,6 +113,9 @@ struct fnet_batch {
   int next = 0;
   int precision = 1;
   int launch = 0;
+  long long capN = 0, capS = 0, capExt = 0;     // allocated capacities (reused by reload)
+  long long res_N = 0, res_S = 0;               // capacity of the result buffers
+  int res_nG = -1, res_nout = -1;
   std::vector<long long> h_off;
   std::vector<int> h_species;
   // inputs
@@ -145,6 +149,19 @@ struct fnet_batch {
   long long npairs = 0;
   int max_nb = 0;
   int* d_maxnb = nullptr;
+  // cell lists of large periodic structures
+  int4* cellinfo = nullptr;    // [S] bins per axis (x, y, z) and use flag (w)
+  int* cell_off = nullptr;     // [S+1] first global cell of each structure
+  int* cell_count = nullptr;   // [ncells] (reused as the scatter cursor)
+  int* cell_start = nullptr;   // [ncells+1]
+  int* cell_atoms = nullptr;   // [N]
+  int* atom_cell = nullptr;    // [N]
+  int4* atom_m = nullptr;      // [N] integer part of the fractional coordinates, local bin
+  int* cell_scan_tmp = nullptr;
+  long long ncells = 0;
+  std::vector<int> h_cellinfo; // [4*S]
+  std::vector<double> h_lat;   // [S][9]
+  std::vector<int> h_periodic; // [S]
   // results
   void* G = nullptr;           // [N][nG] FP64 (accumulated in FP64 in both precisions)
   void* cG = nullptr;          // [N][nout][nG] T
@@ -168,6 +185,13 @@ hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a
 hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s);
 long long scan_tmp_size(long long n);
 
+// launchers (fnet_cells.hip)
+constexpr int kCellMinAtoms = 1024; // structures at least this large use cell lists
+hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s);
+hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
+hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
+                             hipStream_t s);
+
 // launchers (fnet_acsf.hip)
 hipError_t launch_acsf_fwd(fnet_batch* b, const DevModel& m, long long a0, long long a1, int fp64,
                            hipStream_t s);
@@ -182,6 +206,9 @@ hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const
                            const double* at, const double* aw, const double* dpw, double inv_wsum,
                            hipStream_t s);
 hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad, hipStream_t s);
+hipError_t launch_msloss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
+                         const double* at, const double* aw, const double* dpw, double* out,
+                         hipStream_t s);
 hipError_t launch_sd_step(double* params, const double* grad, long long n, double lr,
                           double max_disp, double tol, double* scratch, int nspecies,
                           hipStream_t s);

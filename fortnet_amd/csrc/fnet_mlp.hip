@@ -64,7 +64,7 @@ struct MlpArgs {
   int first_tile;
   int TA;
   double* Eatom;      // [N][nout]
-  void* cG;           // [N][nout][nG] T
+  double* cG;         // [N][nout][nG]
   const double* lossgrad; // [N][nout]
   double* grad;       // [nspecies][nparams]
 };
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpArgs a) {
     for (int idx = threadIdx.x; idx < cnt * a.nG; idx += blockDim.x) {
       const int at = idx % cnt, f = idx / cnt;
       const long long atom = a.perm[start + at];
-      ((T*)a.cG)[(atom * a.nout + t) * a.nG + f] = dcur[at * a.sh.dims[0] + f] * (T)a.zinv[f];
+      a.cG[(atom * a.nout + t) * a.nG + f] = (double)dcur[at * a.sh.dims[0] + f] * a.zinv[f];
     }
     __syncthreads();
   }
@@ -309,6 +309,59 @@ __global__ void k_lossgrad(const double* __restrict__ Eatom, const double* __res
   }
 }
 
+// Mean-squared loss of the batch (msLoss, loss.F90:512-579) in FP64, one block: global targets
+// weighted by w_s * sum(atomic weights of s), atomic targets by w_s * atomic weight; the result
+// is divided by the summed weights.
+__global__ void __launch_bounds__(256) k_msloss(const double* __restrict__ Eatom,
+                                                const double* __restrict__ Estruct,
+                                                const long long* __restrict__ off, int S, int nout,
+                                                int nglob, const double* __restrict__ gt,
+                                                const double* __restrict__ at,
+                                                const double* __restrict__ aw,
+                                                const double* __restrict__ dpw,
+                                                double* __restrict__ out) {
+  __shared__ double rl[256], rn[256];
+  double l = 0.0, nv = 0.0;
+  const int natomic = nout - nglob;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    const double w = dpw ? dpw[s] : 1.0;
+    double saw = 0.0;
+    for (long long i = off[s]; i < off[s + 1]; ++i) {
+      const double awi = aw ? aw[i] : 1.0;
+      saw += awi;
+      if (natomic > 0) {
+        double t = 0.0;
+        for (int o = 0; o < natomic; ++o) {
+          const double d = at[i * natomic + o] - Eatom[i * nout + nglob + o];
+          t += d * d;
+        }
+        l += w * awi * (t / natomic);
+        nv += w * awi;
+      }
+    }
+    if (nglob > 0) {
+      double t = 0.0;
+      for (int o = 0; o < nglob; ++o) {
+        const double d = gt[(long long)s * nglob + o] - Estruct[(long long)s * nout + o];
+        t += d * d;
+      }
+      l += w * saw * (t / nglob);
+      nv += w * saw;
+    }
+  }
+  rl[threadIdx.x] = l;
+  rn[threadIdx.x] = nv;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      rl[threadIdx.x] += rl[threadIdx.x + st];
+      rn[threadIdx.x] += rn[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = rl[0] / rn[0];
+}
+
 // Steepest descent of one species' parameter vector (steepdesc.F90:160-204), FP64.
 __global__ void __launch_bounds__(256) k_sd_step(double* __restrict__ x,
                                                  const double* __restrict__ g, long long n,
@@ -375,7 +428,7 @@ MlpArgs make_args(fnet_batch* b, const DevModel& m, int first_tile) {
   a.first_tile = first_tile;
   a.TA = b->tile_atoms;
   a.Eatom = b->Eatom;
-  a.cG = b->cG;
+  a.cG = (double*)b->cG;
   a.lossgrad = b->lossgrad;
   a.grad = nullptr;
   return a;
@@ -420,6 +473,14 @@ hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const
   k_lossgrad<<<(unsigned)((b->N + 255) / 256), 256, 0, s>>>(b->Eatom, b->Estruct, b->atom_struct,
                                                             b->off, b->N, m.nout, n_global, gt, at,
                                                             aw, dpw, inv_wsum, b->lossgrad);
+  return hipGetLastError();
+}
+
+hipError_t launch_msloss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
+                         const double* at, const double* aw, const double* dpw, double* out,
+                         hipStream_t s) {
+  k_msloss<<<1, 256, 0, s>>>(b->Eatom, b->Estruct, b->off, b->S, m.nout, n_global, gt, at, aw, dpw,
+                             out);
   return hipGetLastError();
 }
 

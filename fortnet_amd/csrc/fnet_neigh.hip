@@ -138,12 +138,14 @@ __global__ void __launch_bounds__(256)
             const int* __restrict__ atom_struct, const StructInfo* __restrict__ sinfo,
             const double* __restrict__ lat, double rc, long long a0, long long a1,
             int* __restrict__ cnt, int* __restrict__ maxnb, const int* __restrict__ rowptr,
-            int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd) {
+            int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd,
+            const int4* __restrict__ cellinfo) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const long long i = a0 + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   if (i >= a1) return;
   const int s = atom_struct[i];
+  if (cellinfo[s].w) return;  // structure handled by the cell-list kernels
   const long long o = off[s];
   const int n = (int)(off[s + 1] - o);
   const int li = (int)(i - o);
@@ -306,7 +308,7 @@ hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long 
   const unsigned grid = (unsigned)((n + 3) / 4);
   k_neigh<false, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
                                                rc, a0, a1, b->cnt, b->d_maxnb, nullptr, nullptr,
-                                               nullptr, nullptr);
+                                               nullptr, nullptr, b->cellinfo);
   return hipGetLastError();
 }
 
@@ -318,11 +320,11 @@ hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a
   if (fp64) {
     k_neigh<true, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
                                                 rc, a0, a1, nullptr, nullptr, b->rowptr, b->nj,
-                                                b->nimg, (double4*)b->nd);
+                                                b->nimg, (double4*)b->nd, b->cellinfo);
   } else {
     k_neigh<true, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
                                                rc, a0, a1, nullptr, nullptr, b->rowptr, b->nj,
-                                               b->nimg, (float4*)b->nd);
+                                               b->nimg, (float4*)b->nd, b->cellinfo);
   }
   return hipGetLastError();
 }

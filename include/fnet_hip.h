@@ -31,6 +31,8 @@ enum fnet_status {
 enum fnet_precision {
   FNET_FP64 = 0, /* everything FP64: the "plain" path, parity ~1e-12 with the FP64 oracle */
   FNET_FP32 = 1, /* ACSF, layers and forces FP32; energy/loss/gradient reductions FP64 */
+  FNET_FP32_NET64 = 2, /* ACSF and forces FP32, element networks FP64 (ill-conditioned
+                          z-scores: the FP32 input gradient limits the force error) */
 };
 
 /* Launch shape. */
@@ -105,6 +107,10 @@ int fnet_params_upload(fnet_ctx* ctx, const double* params);
  * Replaces the per-structure geometry setup of fnetdata.F90:1141-1162. */
 int fnet_batch_upload(fnet_ctx* ctx, const fnet_batch_desc* desc, fnet_batch** out);
 void fnet_batch_destroy(fnet_batch* batch);
+/* Replaces the structures of a resident batch, reusing its device buffers when they are large
+ * enough: the next set of requests of a serving loop.  (No reference counterpart; the i-PI
+ * driver of prg_fnet/fortnet.F90:126-169 re-reads one structure per step.) */
+int fnet_batch_reload(fnet_batch* batch, const fnet_batch_desc* desc);
 /* New coordinates (and lattices, may be NULL to keep) for the same atoms: the serving/MD path. */
 int fnet_batch_update_coords(fnet_batch* batch, const double* coords, const double* latvecs);
 int fnet_batch_set_mode(fnet_batch* batch, int precision, int launch);
@@ -142,10 +148,12 @@ int fnet_run(fnet_ctx* ctx, fnet_batch* batch);
  * g_targets [n_struct][n_global], a_targets [n_atoms][n_out-n_global] (may be NULL when
  * n_out == n_global), atomic_weights [n_atoms] (NULL = 1), dp_weights [n_struct] (NULL = 1).
  * grad [n_species][n_params] receives the FP64 gradient divided by sum(dp_weights);
- * pred [n_atoms][n_out] (may be NULL) the network outputs. */
+ * pred [n_atoms][n_out] (may be NULL) the network outputs; loss (may be NULL) the mse loss of
+ * msLoss (loss.F90:512-579), FP64. */
 int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* batch, int n_global, const double* g_targets,
                          const double* a_targets, const double* atomic_weights,
-                         const double* dp_weights, double* grad, double* pred);
+                         const double* dp_weights, double* grad, double* pred,
+                         double* loss);
 
 /* Steepest-descent update in FP64 on the device (steepdesc.F90:160-204):
  * x -= lr*g, scaled to max_disp if max|lr*g| > max_disp; converged if max|g| < tol.
@@ -156,6 +164,9 @@ int fnet_optimizer_step_sd(fnet_ctx* ctx, double lr, double max_disp, double tol
 
 /* Blocks until the context's stream has drained. */
 int fnet_sync(fnet_ctx* ctx);
+/* Non-blocking: 0 if all work queued on the context has finished, 1 if some is still running,
+ * else an fnet_status (no reference counterpart; used by the serving loop). */
+int fnet_query(fnet_ctx* ctx);
 
 /* Timing of the last fnet_run / fnet_forces call, per stage, in milliseconds (HIP events on
  * the context stream): [0] neighbour list, [1] ACSF forward, [2] networks, [3] adjoint forces,

@@ -32,7 +32,11 @@ def ctx():
 
 def _check(got, want, precision, what):
     if precision == fa.FNET_FP64:
-        assert np.allclose(got, want, atol=ATOL, rtol=RTOL), \
+        # the reference's tolerance, with RTOL taken relative to the largest value: the device
+        # sums in a different order than the reference, and near-zero force components inherit
+        # the rounding of the large terms that cancel in them
+        tol = ATOL + RTOL * max(1.0, float(np.abs(want).max()))
+        assert np.abs(got - want).max() <= tol, \
             "%s: max abs err %.3e" % (what, np.abs(got - want).max())
     else:
         scale = max(1.0, float(np.abs(want).max()))

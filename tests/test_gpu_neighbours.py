@@ -11,6 +11,7 @@ import pytest
 import fortnet_amd as fa
 from fortnet_amd import dataset
 from fortnet_amd import workloads
+from fortnet_amd import workloads as W
 from conftest import load_case
 from oracle import oracle as O
 
@@ -71,6 +72,20 @@ def test_small_cells_with_self_images(ctx, seed):
     for n in (1, 2, 8, 13, 40):
         geos.append(workloads.random_cell(rng, n, 2.33, 2.0, 14))
     _compare(ctx, geos, 6.0 * fa.AA__BOHR)
+
+
+def test_cell_list_path_exact(ctx):
+    """Structures of >= 1024 atoms with >= 3 bins per axis go through the cell-list kernels; the
+    sorted rows must still be the reference's entries in the reference's order, including atoms
+    moved out of the cell by lattice vectors (unwrapped coordinates)."""
+    g = W.si_diamond(6, 6, 4, 0.1, 11)
+    assert g.n >= 1024
+    xyz = g.coords.copy()
+    L = g.latvecs
+    xyz[::7] += L[0]
+    xyz[3::11] -= L[2]
+    g2 = fa.Geometry(xyz, g.atnum, g.species, L)
+    _compare(ctx, [g, g2], 6.0 * fa.AA__BOHR)
 
 
 def test_triclinic_and_unwrapped(ctx):
