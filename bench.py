@@ -30,6 +30,7 @@ import fortnet_amd as fa  # noqa: E402
 from fortnet_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def algorithmic_bytes(stage, N, P, nG):
@@ -43,6 +44,43 @@ def algorithmic_bytes(stage, N, P, nG):
     if stage == "networks":
         return 4 * nG * N + 4 * N + 4 * nG * N
     return 8 * N
+
+
+def acsf_flops(batch, geos, ns):
+    """Algorithmic flops of one ACSF forward launch over the batch, in the reference's loop shape
+    (acsf.F90:1287-1492): per neighbour entry 8 (distance, list cutoff) + 4 per radial function
+    whose species filter admits it; per ordered pair (j, k) of an angular function's two species
+    lists (j == k included, as the reference's double loop) 10 shared by the functions of one
+    (type, cutoff, eta, species pair) group, + 8 for the G4 r_jk cutoff, + 6 per function.
+    The adjoint pass is counted as 2.5x the forward."""
+    rp, j, _ = batch.neighbours()
+    N = len(rp) - 1
+    Z = np.concatenate([g.atnum for g in geos])
+    owner = np.repeat(np.arange(N), np.diff(rp))
+    zj = Z[j]
+    n_all = np.diff(rp).astype(np.float64)
+    n_z = {int(z): np.bincount(owner[zj == z], minlength=N).astype(np.float64)
+           for z in np.unique(Z)}
+
+    def count(z):
+        return n_all if z == 0 else n_z.get(int(z), np.zeros(N))
+
+    g = ns.g
+    fl = 8.0 * n_all.sum()
+    groups = {}
+    for f in range(len(g)):
+        t = int(g.type[f])
+        z1 = int(g.z1[f]) if g.z1 is not None else 0
+        z2 = int(g.z2[f]) if g.z2 is not None else 0
+        if t <= 3:
+            fl += 4.0 * count(z1).sum()
+        else:
+            key = (t, float(g.rcut[f]), float(g.eta[f]), z1, z2)
+            groups[key] = groups.get(key, 0) + 1
+    for (t, _, _, z1, z2), nmem in groups.items():
+        pairs = (count(z1) * count(z2)).sum()
+        fl += pairs * (10.0 + (8.0 if t == 4 else 0.0) + 6.0 * nmem)
+    return fl
 
 
 def cpu_baseline(geos, ns, budget_s=20.0):
@@ -179,6 +217,11 @@ def main():
     roofline = {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "algorithmic_bytes": ab, "kernel_ms": float(stage[k])}
+    if names[k] in ("acsf", "adjoint"):
+        fl = acsf_flops(b, geos, ns) * (2.5 if names[k] == "adjoint" else 1.0)
+        tf = fl / (stage[k] * 1e-3) / 1e12
+        roofline["valu"] = {"algorithmic_flops": fl, "achieved": tf, "peak": FP32_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS}
 
     out = {
         "metric": "atoms/s energy+forces, batched vs plain GPU @1e-5 eV/atom; serve p50/p99 ms",

@@ -1,10 +1,10 @@
 // ACSF values and their adjoint (force) pass on gfx950.
 //
 // One wavefront per centre atom.  The centre's CSR row of the neighbour list (image displacement
-// and distance per entry) is staged once into the wave's LDS slice with coalesced loads, with
-// the cutoff function of the list cutoff and per-element index lists built by ballots, so every
-// species-resolved list of buildGFunctionNeighborlists (acsf.F90:943-1066) is a slice of LDS
-// instead of a rebuilt neighbour list.
+// and distance per entry) is staged once into the wave's LDS slice with coalesced loads, ordered
+// by element (a counting pass and a placing pass, both with ballots), so every species-resolved
+// neighbour list of buildGFunctionNeighborlists (acsf.F90:943-1066) is a contiguous range of the
+// slice instead of a rebuilt neighbour list.
 //
 //  * radial G1/G2/G3 (acsf.F90:1287-1373): lanes = (entry stride group, function), partial sums
 //    combined through LDS;
@@ -14,7 +14,7 @@
 //    once with weight 2 when both lists are the same (the ordered double sum with its j == k
 //    terms, acsf.F90:1420-1432), or the full list1 x list2 product otherwise.  With this order
 //    the 64 lanes of one step touch 64 different entries, which keeps the adjoint's LDS
-//    accumulation free of same-address collisions;
+//    accumulation free of same-address collisions, and the walk needs no division per step;
 //  * the adjoint pass computes F = -dE/dR directly from c = dE/dG (the network's input gradient
 //    times the z-score factor): the derivative of every function with respect to each entry's
 //    position (gFuncGrad, acsf.F90:1496-1670) is weighted by c and summed per entry, then scattered
@@ -22,7 +22,9 @@
 //    dense array, but without forming it and with periodic images summed exactly.
 //
 // Arithmetic is T (double or float); sums over neighbours and pairs, the values G and the forces
-// are accumulated in FP64 in both modes.
+// are accumulated in FP64 in both modes.  FP64 evaluates the elementary functions with libm in the
+// reference's operation order; FP32 uses the hardware instructions (v_exp_f32, v_cos_f32,
+// v_sin_f32, v_sqrt_f32, v_rcp_f32: about 1 ulp, cos/sin to ~1e-6 absolute on [0, pi]).
 #include "fnet_internal.hpp"
 
 namespace fnet {
@@ -50,18 +52,18 @@ __device__ __forceinline__ float fsin(float x) { return __sinf(x); }
 __device__ __forceinline__ double fsin(double x) { return sin(x); }
 __device__ __forceinline__ float fpow(float x, float y) { return powf(x, y); }
 __device__ __forceinline__ double fpow(double x, double y) { return pow(x, y); }
-
-// x^n for a wave-uniform integer n >= 0, else x^y
-template <typename T>
-__device__ __forceinline__ T xpow(T x, int n, T y) {
-  if (n < 0) return fpow(x, y);
-  T r = (T)1;
-  while (n) {
-    if (n & 1) r *= x;
-    x *= x;
-    n >>= 1;
-  }
-  return r;
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fdiv(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+__device__ __forceinline__ double fdiv(double x, double y) { return x / y; }
+// argument pi r / rc of the cosine cutoff (FP32: with the wave-uniform pi / rc hoisted)
+__device__ __forceinline__ float cutarg(float r, float pi, float rc, float pirc) {
+  return r * pirc;
+}
+__device__ __forceinline__ double cutarg(double r, double pi, double rc, double pirc) {
+  return pi * r / rc;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -87,24 +89,24 @@ struct AcsfArgs {
   double rclist;
   long long a0, a1;
   int maxnb;
-  double* G;      // [N][nG] (forward)
+  double* G;        // [N][nG] (forward)
   const double* cG; // [N][nout][nG] (adjoint)
-  double* F;      // [N][nout][3] (adjoint)
+  double* F;        // [N][nout][3] (adjoint)
 };
 
-// Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry: the
-// displacement and distance d, the pair-loop factors pa = (1/r, exp(-eta r^2) fc, atom-id
-// prefactor, radial derivative), the list-cutoff fc and fc', the neighbour index and element
-// code, the adjoint accumulator (adjoint only) and the per-element index lists.
+// Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry, in element
+// order: the displacement and distance d, the pair-loop factors pa = (1/r, exp(-eta r^2) fc,
+// atom-id prefactor, radial derivative), the list-cutoff fc and fc', the FP64 adjoint
+// accumulator (adjoint only), the neighbour index and element code; then the radial partial
+// sums and the start and length of each element's range.
 template <typename T, bool ADJ>
-__host__ __device__ size_t slice_bytes(int M, int nZ) {
+__host__ __device__ size_t slice_bytes(int M) {
   size_t b = 2 * sizeof(typename Vec4<T>::type) * M;  // d, pa
-  b += 2 * sizeof(T) * M;                             // fc, dfc
-  if (ADJ) b += 3 * sizeof(T) * M;                    // gs
-  b += 2 * sizeof(int) * M;                           // j, zc
   b += 64 * sizeof(double);                           // radial partial sums
-  b += sizeof(int) * kMaxZ;                           // nz
-  b += sizeof(unsigned short) * (size_t)nZ * M;       // idx
+  b += 2 * sizeof(T) * M;                             // fc, dfc
+  if (ADJ) b += 3 * sizeof(double) * M;               // gs
+  b += 2 * sizeof(int) * M;                           // j, zc
+  b += 2 * sizeof(int) * (kMaxZ + 1);                 // st, nz
   return (b + 31) & ~size_t(31);
 }
 
@@ -114,11 +116,10 @@ struct Slice {
   V* d;
   V* pa;
   T *fc, *dfc;
-  T* gs;
+  double* gs;  // FP64: ds_add_f64 measured faster than ds_add_f32 here, and exact to more bits
   int *j, *zc;
   double* red;
-  int* nz;
-  unsigned short* idx;
+  int *st, *nz;
 };
 
 template <typename T, bool ADJ>
@@ -137,16 +138,16 @@ __device__ Slice<T> make_slice(unsigned char* p, int M) {
   p += sizeof(T) * M;
   s.gs = nullptr;
   if (ADJ) {
-    s.gs = (T*)p;
-    p += 3 * sizeof(T) * M;
+    s.gs = (double*)p;
+    p += 3 * sizeof(double) * M;
   }
   s.j = (int*)p;
   p += sizeof(int) * M;
   s.zc = (int*)p;
   p += sizeof(int) * M;
+  s.st = (int*)p;
+  p += sizeof(int) * (kMaxZ + 1);
   s.nz = (int*)p;
-  p += sizeof(int) * kMaxZ;
-  s.idx = (unsigned short*)p;
   return s;
 }
 
@@ -154,65 +155,254 @@ __device__ __forceinline__ double ext_q(const double* ext, int next, int atomid,
   return atomid > 0 ? ext[atom * next + (atomid - 1)] : 1.0;
 }
 
-// Pair enumeration of one group.  Same lists: the diagonal (d = 0, weight 1), then every
-// unordered pair once as (j, j + d mod L), d = 1 .. L/2, weight 2.  Different lists: every pair of
-// list1 x list2 once; the longer list runs over the lanes and k = j + d mod (shorter length).
-struct PairIter {
-  const unsigned short* la;
-  const unsigned short* lb;
-  int LA, LB, d, j, dmax;
-  bool same, even, done;
+// Off-diagonal pair walk of one group over the element ranges [A0, A0 + LA) and [B0, B0 + LB).
+// Step index t = 64 s + lane + t0 maps to (j, d) = (t mod LA, t div LA) and k = (j + d) mod LB;
+// j and k advance by wave-uniform increments, so a step needs no division.  Same lists (SAME):
+// t0 = LA skips the diagonal d = 0, which the callers treat on their own; every unordered pair
+// then comes once (weight 2): d = 1 .. (LA - 1) / 2, plus half of d = LA / 2 when LA is even.
+// Different lists: the longer list is indexed by j, every pair once (weight 1).
+template <bool SAME>
+struct PairWalk {
+  int A0, B0, LA, LB;
+  int j, k, t, tend;
+  int rs, kstep, kwrap;
 
-  __device__ __forceinline__ void init(int lane, bool s, int L1, int L2, const unsigned short* l1,
-                                       const unsigned short* l2) {
-    same = s;
-    if (same) {
-      LA = LB = L1;
-      la = lb = l1;
+  __device__ __forceinline__ void init(int lane, int a0, int l1, int b0, int l2) {
+    if (!SAME && l2 > l1) {
+      A0 = b0;
+      LA = l2;
+      B0 = a0;
+      LB = l1;
     } else {
-      const bool sw = L2 > L1;
-      LA = sw ? L2 : L1;
-      LB = sw ? L1 : L2;
-      la = sw ? l2 : l1;
-      lb = sw ? l1 : l2;
+      A0 = a0;
+      LA = l1;
+      B0 = b0;
+      LB = l2;
     }
-    dmax = (LA - 1) / 2;
-    even = (LA & 1) == 0;
-    d = 0;
-    j = lane;
-    done = (LA <= 0 || LB <= 0);
-    if (!done)
-      while (j >= LA) {
-        j -= LA;
-        ++d;
-      }
+    j = k = rs = kstep = kwrap = 0;
+    t = lane;
+    tend = 0;
+    if (LA <= 0 || LB <= 0) return;
+    int d;
+    if (SAME) {
+      tend = LA * ((LA - 1) / 2 + 1) + (((LA & 1) == 0) ? (LA >> 1) : 0);
+      t = lane + LA;
+      j = t % LA;
+      d = t / LA;
+    } else {
+      tend = LA * LB;
+      j = lane % LA;
+      d = lane / LA;
+    }
+    k = (j + d) % LB;
+    const int qs = 64 / LA;
+    rs = 64 % LA;
+    kstep = (rs + qs) % LB;
+    kwrap = ((1 - LA) % LB + LB) % LB;
   }
 
-  __device__ __forceinline__ bool next(int& ej, int& ek, bool& diag, int& w) {
-    if (done) return false;
-    if (same) {
-      if (d > dmax && !(even && d == dmax + 1 && j < (LA >> 1))) {
-        done = true;
-        return false;
-      }
-    } else if (d >= LB) {
-      done = true;
-      return false;
-    }
-    int k = j + d;
-    while (k >= LB) k -= LB;
-    ej = la ? (int)la[j] : j;
-    ek = lb ? (int)lb[k] : k;
-    diag = same && d == 0;
-    w = (same && d != 0) ? 2 : 1;
-    j += 64;
-    while (j >= LA) {
+  __device__ __forceinline__ bool next(int& ej, int& ek) {
+    if (t >= tend) return false;
+    ej = A0 + j;
+    ek = B0 + k;
+    t += 64;
+    j += rs;
+    k += kstep;
+    k = (k >= LB) ? k - LB : k;
+    if (j >= LA) {
       j -= LA;
-      ++d;
+      k += kwrap;
+      k = (k >= LB) ? k - LB : k;
     }
     return true;
   }
 };
+
+// x^n, n >= 0 wave-uniform: the common exponents of ACSF sets as straight-line products (the
+// switch is on a scalar, so only the taken product costs vector instructions), others by square
+// and multiply
+template <typename T>
+__device__ __forceinline__ T powi(T x, int n) {
+  switch (n) {
+    case 0: return (T)1;
+    case 1: return x;
+    case 2: return x * x;
+    case 3: return x * x * x;
+    case 4: {
+      const T y = x * x;
+      return y * y;
+    }
+    case 7: {
+      const T y = x * x;
+      const T z = y * y;
+      return z * y * x;
+    }
+    case 8: {
+      const T y = x * x;
+      const T z = y * y;
+      return z * z;
+    }
+    case 15: {
+      const T y = x * x;
+      const T z = y * y;
+      const T u = z * z;
+      return u * z * y * x;
+    }
+    case 16: {
+      const T y = x * x;
+      const T z = y * y;
+      const T u = z * z;
+      return u * u;
+    }
+    default: {
+      T r = (T)1;
+      while (n) {
+        if (n & 1) r *= x;
+        x *= x;
+        n >>= 1;
+      }
+      return r;
+    }
+  }
+}
+
+// Members of one angular group (wave-uniform).  Padding members (q >= nmem) have lam = 0,
+// xint = 0 and A = 0, so they can be evaluated unconditionally and contribute nothing.
+template <typename T, int MQ>
+struct Members {
+  T lam[MQ], xi[MQ];
+  int xint[MQ];
+  T pdiag[MQ];  // (1 + lam)^xi: the j == k terms, where cos(theta) = 1
+  T A[MQ];      // adjoint: dE/dG_q times 2^(1 - xi_q)
+  T AXL[MQ];    // adjoint: A xi lam
+};
+
+// Forward: off-diagonal pairs of one group, FP64 partial sums per member in acc.
+template <typename T, int MQ, bool SAME, bool G4, bool IPOW>
+__device__ __forceinline__ void pairs_fwd(const Slice<T>& sl, int lane, int A0, int LA, int B0,
+                                          int LB, const Members<T, MQ>& mb, T rc, T eta, T pi,
+                                          T pirc, double (&acc)[MQ]) {
+  using V = typename Vec4<T>::type;
+  PairWalk<SAME> it;
+  it.init(lane, A0, LA, B0, LB);
+  int ej, ek;
+  while (it.next(ej, ek)) {
+    const V dj = sl.d[ej], dk = sl.d[ek];
+    const V pj = sl.pa[ej], pk = sl.pa[ek];
+    T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
+    c = fmin(fmax(c, (T)-1), (T)1);
+    T base = (SAME ? (T)2 : (T)1) * pj.y * pk.y;
+    if (G4) {
+      const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
+      const T r2 = rx * rx + ry * ry + rz * rz;
+      const T rjk = fsqrt(r2);
+      const T fcjk = (T)0.5 * pj.z * pk.z * (fcos(cutarg(rjk, pi, rc, pirc)) + (T)1);
+      const T f = fexp(-eta * r2) * fcjk;
+      base = (rjk <= rc) ? base * f : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < MQ; ++q) {
+      const T P = (T)1 + mb.lam[q] * c;
+      const T pw = IPOW ? powi(P, mb.xint[q]) : fpow(P, mb.xi[q]);
+      acc[q] += (double)(base * pw);
+    }
+  }
+}
+
+// Adjoint: off-diagonal pairs of one group, gradients accumulated per entry in sl.gs.
+template <typename T, int MQ, bool SAME, bool G4, bool IPOW>
+__device__ __forceinline__ void pairs_adj(const Slice<T>& sl, int lane, int A0, int LA, int B0,
+                                          int LB, const Members<T, MQ>& mb, T rc, T eta, T pi,
+                                          T pirc) {
+  using V = typename Vec4<T>::type;
+  PairWalk<SAME> it;
+  it.init(lane, A0, LA, B0, LB);
+  const T w = SAME ? (T)2 : (T)1;
+  int ej, ek;
+  while (it.next(ej, ek)) {
+    const V dj = sl.d[ej], dk = sl.d[ek];
+    const V pj = sl.pa[ej], pk = sl.pa[ek];
+    const T irj = pj.x, irk = pk.x;
+    const T Ej = pj.y, Epj = pj.w, Ek = pk.y, Epk = pk.w;
+    T H = 1, Hp = 0, irjk = 0;
+    T rx = 0, ry = 0, rz = 0;
+    if (G4) {
+      rx = dj.x - dk.x;
+      ry = dj.y - dk.y;
+      rz = dj.z - dk.z;
+      const T r2 = rx * rx + ry * ry + rz * rz;
+      const T rjk = fsqrt(r2);
+      if (!(rjk <= rc)) continue;
+      const T qq = pj.z * pk.z;
+      const T x = cutarg(rjk, pi, rc, pirc);
+      const T ex = fexp(-eta * r2);
+      const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
+      const T dfc = -(T)0.5 * qq * pirc * fsin(x);
+      H = ex * fc;
+      Hp = ex * (dfc - (T)2 * eta * rjk * fc);
+      irjk = frcp(rjk);
+    }
+    const T ujx = dj.x * irj, ujy = dj.y * irj, ujz = dj.z * irj;
+    const T ukx = dk.x * irk, uky = dk.y * irk, ukz = dk.z * irk;
+    const T c = fmin(fmax(ujx * ukx + ujy * uky + ujz * ukz, (T)-1), (T)1);
+    T S0 = 0, S1 = 0;
+#pragma unroll
+    for (int q = 0; q < MQ; ++q) {
+      const T P = (T)1 + mb.lam[q] * c;
+      T pw, pm1;
+      if (IPOW) {
+        const int n = mb.xint[q];
+        pm1 = n > 0 ? powi(P, n - 1) : (T)0;
+        pw = n > 0 ? pm1 * P : (T)1;
+      } else {
+        pw = fpow(P, mb.xi[q]);
+        pm1 = fpow(P, mb.xi[q] - (T)1);
+      }
+      S0 += mb.A[q] * pw;
+      S1 += mb.AXL[q] * pm1;
+    }
+    const T B = Ej * Ek * H;
+    const T cj = w * S1 * B * irj, ck = w * S1 * B * irk;
+    const T aj = w * S0 * Epj * Ek * H, ak = w * S0 * Ej * Epk * H;
+    T gjx = cj * (ukx - c * ujx) + aj * ujx;
+    T gjy = cj * (uky - c * ujy) + aj * ujy;
+    T gjz = cj * (ukz - c * ujz) + aj * ujz;
+    T gkx = ck * (ujx - c * ukx) + ak * ukx;
+    T gky = ck * (ujy - c * uky) + ak * uky;
+    T gkz = ck * (ujz - c * ukz) + ak * ukz;
+    if (G4) {
+      const T h = w * S0 * Ej * Ek * Hp * irjk;
+      gjx += h * rx;
+      gjy += h * ry;
+      gjz += h * rz;
+      gkx -= h * rx;
+      gky -= h * ry;
+      gkz -= h * rz;
+    }
+    atomicAdd(&sl.gs[3 * ej + 0], (double)gjx);
+    atomicAdd(&sl.gs[3 * ej + 1], (double)gjy);
+    atomicAdd(&sl.gs[3 * ej + 2], (double)gjz);
+    atomicAdd(&sl.gs[3 * ek + 0], (double)gkx);
+    atomicAdd(&sl.gs[3 * ek + 1], (double)gky);
+    atomicAdd(&sl.gs[3 * ek + 2], (double)gkz);
+  }
+}
+
+template <typename T, int MQ, bool SAME, bool G4>
+__device__ __forceinline__ void pairs_fwd_p(bool ipow, const Slice<T>& sl, int lane, int A0,
+                                            int LA, int B0, int LB, const Members<T, MQ>& mb,
+                                            T rc, T eta, T pi, T pirc, double (&acc)[MQ]) {
+  if (ipow) pairs_fwd<T, MQ, SAME, G4, true>(sl, lane, A0, LA, B0, LB, mb, rc, eta, pi, pirc, acc);
+  else pairs_fwd<T, MQ, SAME, G4, false>(sl, lane, A0, LA, B0, LB, mb, rc, eta, pi, pirc, acc);
+}
+
+template <typename T, int MQ, bool SAME, bool G4>
+__device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int lane, int A0,
+                                            int LA, int B0, int LB, const Members<T, MQ>& mb,
+                                            T rc, T eta, T pi, T pirc) {
+  if (ipow) pairs_adj<T, MQ, SAME, G4, true>(sl, lane, A0, LA, B0, LB, mb, rc, eta, pi, pirc);
+  else pairs_adj<T, MQ, SAME, G4, false>(sl, lane, A0, LA, B0, LB, mb, rc, eta, pi, pirc);
+}
 
 template <typename T, bool ADJ, int MQ>
 __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
@@ -223,44 +413,76 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
   const long long i = a.a0 + (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (i >= a.a1) return;
   const int M = a.maxnb;
-  Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M, a.nZ) * wave, M);
+  Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M) * wave, M);
 
   const int b = a.rowptr[i];
   const int n = a.rowptr[i + 1] - b;
   const V* nd = (const V*)a.nd + b;
   const T pi = (T)kPi;
   const T rcl = (T)a.rclist;
+  const T pircl = pi / rcl;
+  const int nZ = a.nZ;  // element slots 0 .. nZ - 1; slot nZ: elements no function names
 
-  // ---- stage the centre's entries, list-cutoff fc, per-element index lists ----
-  int running[kMaxZ];
+  // ---- stage the centre's entries in element order with the list-cutoff fc ----
+  int cnt[kMaxZ + 1];
 #pragma unroll
-  for (int c = 0; c < kMaxZ; ++c) running[c] = 0;
+  for (int c = 0; c <= kMaxZ; ++c) cnt[c] = 0;
   for (int e0 = 0; e0 < n; e0 += 64) {
     const int e = e0 + lane;
-    int zc = -1;
+    int key = -1;
     if (e < n) {
-      const int jj = a.nj[b + e];
-      const V d = nd[e];
-      sl.d[e] = d;
-      sl.j[e] = jj;
-      zc = a.zcode[jj];
-      sl.zc[e] = zc;
-      const T x = pi * d.w / rcl;
-      sl.fc[e] = (T)0.5 * (fcos(x) + (T)1);
-      sl.dfc[e] = -(T)0.5 * (pi / rcl) * fsin(x);
+      const int zc = a.zcode[a.nj[b + e]];
+      key = zc >= 0 ? zc : nZ;
     }
 #pragma unroll
-    for (int c = 0; c < kMaxZ; ++c) {
-      if (c < a.nZ) {
-        const unsigned long long m = __ballot(e < n && zc == c);
-        if (e < n && zc == c) sl.idx[c * M + running[c] + __popcll(m & lanemask_lt(lane))] = e;
-        running[c] += __popcll(m);
-      }
+    for (int c = 0; c <= kMaxZ; ++c)
+      if (c <= nZ) cnt[c] += __popcll(__ballot(key == c));
+  }
+  int start[kMaxZ + 1];
+  {
+    int s = 0;
+#pragma unroll
+    for (int c = 0; c <= kMaxZ; ++c) {
+      start[c] = s;
+      if (c <= nZ) s += cnt[c];
     }
   }
 #pragma unroll
-  for (int c = 0; c < kMaxZ; ++c)
-    if (c == lane) sl.nz[c] = running[c];
+  for (int c = 0; c <= kMaxZ; ++c)
+    if (c == lane && c <= nZ) {
+      sl.st[c] = start[c];
+      sl.nz[c] = cnt[c];
+    }
+  int placed[kMaxZ + 1];
+#pragma unroll
+  for (int c = 0; c <= kMaxZ; ++c) placed[c] = 0;
+  for (int e0 = 0; e0 < n; e0 += 64) {
+    const int e = e0 + lane;
+    int key = -1, jj = 0;
+    if (e < n) {
+      jj = a.nj[b + e];
+      const int zc = a.zcode[jj];
+      key = zc >= 0 ? zc : nZ;
+    }
+    int pos = 0;
+#pragma unroll
+    for (int c = 0; c <= kMaxZ; ++c) {
+      if (c <= nZ) {
+        const unsigned long long m = __ballot(key == c);
+        if (key == c) pos = start[c] + placed[c] + __popcll(m & lanemask_lt(lane));
+        placed[c] += __popcll(m);
+      }
+    }
+    if (e < n) {
+      const V d = nd[e];
+      sl.d[pos] = d;
+      sl.j[pos] = jj;
+      sl.zc[pos] = key < nZ ? key : -1;
+      const T x = cutarg(d.w, pi, rcl, pircl);
+      sl.fc[pos] = (T)0.5 * (fcos(x) + (T)1);
+      sl.dfc[pos] = -(T)0.5 * pircl * fsin(x);
+    }
+  }
   wave_sync();
 
   const int nouts = ADJ ? a.nout : 1;
@@ -277,6 +499,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
         const RadialFn& fn = a.radial[f0 + fl];
         const bool shared = (fn.rc == a.rclist) && fn.atomid == 0;
         const T rc = (T)fn.rc, eta = (T)fn.eta, rs = (T)fn.rs, kap = (T)fn.kappa;
+        const T pirc = pi / rc;
         const T qi = (T)ext_q(a.ext, a.next, fn.atomid, i);
         double acc = 0.0;
         if (act) {
@@ -289,7 +512,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
                 fcv = sl.fc[e];
               } else {
                 const T qj = (T)ext_q(a.ext, a.next, fn.atomid, sl.j[e]);
-                fcv = (T)0.5 * qi * qj * (fcos(pi * r / rc) + (T)1);
+                fcv = (T)0.5 * qi * qj * (fcos(cutarg(r, pi, rc, pirc)) + (T)1);
               }
               T term;
               if (fn.type == 1) term = fcv;
@@ -327,11 +550,12 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
             dfc = sl.dfc[e];
           } else {
             const T rc = (T)fn.rc;
+            const T pirc = pi / rc;
             const T qq = (T)(ext_q(a.ext, a.next, fn.atomid, i) *
                              ext_q(a.ext, a.next, fn.atomid, sl.j[e]));
-            const T x = pi * r / rc;
+            const T x = cutarg(r, pi, rc, pirc);
             fc = (T)0.5 * qq * (fcos(x) + (T)1);
-            dfc = -(T)0.5 * qq * (pi / rc) * fsin(x);
+            dfc = -(T)0.5 * qq * pirc * fsin(x);
           }
           T g;
           if (fn.type == 1) {
@@ -345,7 +569,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
           }
           s += cf * g;
         }
-        const T sr = s / r;
+        const T sr = fdiv(s, r);
         sl.gs[3 * e + 0] = sr * d.x;
         sl.gs[3 * e + 1] = sr * d.y;
         sl.gs[3 * e + 2] = sr * d.z;
@@ -368,12 +592,13 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
         if (!any) continue;
       }
       const T rc = (T)grp.rc, eta = (T)grp.eta;
+      const T pirc = pi / rc;
       const bool g4 = grp.type == 4;
       const bool same = grp.same != 0;
       const bool shared = (grp.rc == a.rclist) && grp.atomid == 0;
-      const unsigned short* l1 = grp.zc1 >= 0 ? sl.idx + grp.zc1 * M : nullptr;
-      const unsigned short* l2 = grp.zc2 >= 0 ? sl.idx + grp.zc2 * M : nullptr;
+      const int A0 = grp.zc1 >= 0 ? sl.st[grp.zc1] : 0;
       const int L1 = grp.zc1 >= 0 ? sl.nz[grp.zc1] : n;
+      const int B0 = grp.zc2 >= 0 ? sl.st[grp.zc2] : 0;
       const int L2 = grp.zc2 >= 0 ? sl.nz[grp.zc2] : n;
       const T qi = (T)ext_q(a.ext, a.next, grp.atomid, i);
       // per-entry radial factor exp(-eta r^2) fc(r) of this group (and its r-derivative)
@@ -389,16 +614,16 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
             dfc = sl.dfc[e];
           } else {
             qj = (T)ext_q(a.ext, a.next, grp.atomid, sl.j[e]);
-            const T x = pi * r / rc;
+            const T x = cutarg(r, pi, rc, pirc);
             fc = (T)0.5 * qi * qj * (fcos(x) + (T)1);
-            dfc = -(T)0.5 * qi * qj * (pi / rc) * fsin(x);
+            dfc = -(T)0.5 * qi * qj * pirc * fsin(x);
           }
           const T ex = fexp(-eta * r * r);
           E = ex * fc;
           Ep = ex * (dfc - (T)2 * eta * r * fc);
         }
         V pa;
-        pa.x = (T)1 / r;
+        pa.x = frcp(r);
         pa.y = E;
         pa.z = qj;
         pa.w = Ep;
@@ -406,44 +631,38 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
       }
       wave_sync();
 
-      int xint[MQ];
-      T xi[MQ], lam[MQ];
+      Members<T, MQ> mb;
+      bool ipow = true;
 #pragma unroll
       for (int q = 0; q < MQ; ++q) {
-        xint[q] = (q < nm) ? grp.xint[q] : 0;
-        xi[q] = (q < nm) ? (T)grp.xi[q] : (T)0;
-        lam[q] = (q < nm) ? (T)grp.lam[q] : (T)0;
+        const bool real = q < nm;
+        mb.xint[q] = real ? grp.xint[q] : 0;
+        mb.xi[q] = real ? (T)grp.xi[q] : (T)0;
+        mb.lam[q] = real ? (T)grp.lam[q] : (T)0;
+        ipow = ipow && mb.xint[q] >= 0;
+        mb.pdiag[q] = real ? (T)grp.pdiag[q] : (T)0;
+        mb.A[q] = ADJ ? A[q] : (T)0;
+        mb.AXL[q] = ADJ ? A[q] * mb.xi[q] * mb.lam[q] : (T)0;
       }
 
       if (!ADJ) {
         double acc[MQ];
 #pragma unroll
         for (int q = 0; q < MQ; ++q) acc[q] = 0.0;
-        PairIter it;
-        it.init(lane, same, L1, L2, l1, l2);
-        int ej, ek, wi;
-        bool diag;
-        while (it.next(ej, ek, diag, wi)) {
-          const T w = (T)wi;
-          const V dj = sl.d[ej], dk = sl.d[ek];
-          const V pj = sl.pa[ej], pk = sl.pa[ek];
-          T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
-          c = fmin(fmax(c, (T)-1), (T)1);
-          T base = w * pj.y * pk.y;
-          if (g4) {
-            const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
-            const T r2 = rx * rx + ry * ry + rz * rz;
-            const T rjk = sqrt(r2);
-            if (rjk <= rc) {
-              const T fcjk = (T)0.5 * pj.z * pk.z * (fcos(pi * rjk / rc) + (T)1);
-              base *= fexp(-eta * r2) * fcjk;
-            } else {
-              base = 0;
-            }
-          }
+        if (same) {
+          // j == k terms: cos(theta) = 1, r_jk = 0, fc(0) = 1 (acsf.F90:1420-1432)
+          for (int e = lane; e < L1; e += 64) {
+            const V pj = sl.pa[A0 + e];
+            T base = pj.y * pj.y;
+            if (g4) base *= pj.z * pj.z;
 #pragma unroll
-          for (int q = 0; q < MQ; ++q)
-            if (q < nm) acc[q] += (double)(base * xpow((T)1 + lam[q] * c, xint[q], xi[q]));
+            for (int q = 0; q < MQ; ++q) acc[q] += (double)(base * mb.pdiag[q]);
+          }
+          if (g4) pairs_fwd_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+          else pairs_fwd_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+        } else {
+          if (g4) pairs_fwd_p<T, MQ, false, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+          else pairs_fwd_p<T, MQ, false, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
         }
 #pragma unroll
         for (int q = 0; q < MQ; ++q) {
@@ -453,97 +672,27 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
           }
         }
       } else {
-        // diagonal coefficient: P = 1 + lambda at c = 1 (acsf.F90:1580-1585)
-        T S0d = 0;
+        if (same) {
+          // j == k terms: only the radial factors depend on r_j; P = 1 + lambda at c = 1
+          // (acsf.F90:1580-1585).  Each lane owns its entries here, so the update is plain.
+          T S0d = 0;
 #pragma unroll
-        for (int q = 0; q < MQ; ++q)
-          if (q < nm) S0d += A[q] * xpow((T)1 + lam[q], xint[q], xi[q]);
-        PairIter it;
-        it.init(lane, same, L1, L2, l1, l2);
-        int ej, ek, wi;
-        bool diag;
-        while (it.next(ej, ek, diag, wi)) {
-          const T w = (T)wi;
-          const V dj = sl.d[ej];
-          const V pj = sl.pa[ej];
-          const T irj = pj.x;
-          const T Ej = pj.y, Epj = pj.w;
-          if (diag) {
+          for (int q = 0; q < MQ; ++q) S0d += mb.A[q] * mb.pdiag[q];
+          for (int e = lane; e < L1; e += 64) {
+            const int ej = A0 + e;
+            const V dj = sl.d[ej];
+            const V pj = sl.pa[ej];
             const T H = g4 ? pj.z * pj.z : (T)1;
-            const T gsc = S0d * (T)2 * Ej * Epj * H * irj;
-            atomicAdd(&sl.gs[3 * ej + 0], gsc * dj.x);
-            atomicAdd(&sl.gs[3 * ej + 1], gsc * dj.y);
-            atomicAdd(&sl.gs[3 * ej + 2], gsc * dj.z);
-            continue;
+            const T gsc = S0d * (T)2 * pj.y * pj.w * H * pj.x;
+            sl.gs[3 * ej + 0] += gsc * dj.x;
+            sl.gs[3 * ej + 1] += gsc * dj.y;
+            sl.gs[3 * ej + 2] += gsc * dj.z;
           }
-          const V dk = sl.d[ek];
-          const V pk = sl.pa[ek];
-          const T irk = pk.x;
-          const T Ek = pk.y, Epk = pk.w;
-          T H = 1, Hp = 0, rjk = 1;
-          T rx = 0, ry = 0, rz = 0;
-          if (g4) {
-            rx = dj.x - dk.x;
-            ry = dj.y - dk.y;
-            rz = dj.z - dk.z;
-            const T r2 = rx * rx + ry * ry + rz * rz;
-            rjk = sqrt(r2);
-            if (!(rjk <= rc)) continue;
-            const T qq = pj.z * pk.z;
-            const T x = pi * rjk / rc;
-            const T ex = fexp(-eta * r2);
-            const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
-            const T dfc = -(T)0.5 * qq * (pi / rc) * fsin(x);
-            H = ex * fc;
-            Hp = ex * (dfc - (T)2 * eta * rjk * fc);
-          }
-          const T ujx = dj.x * irj, ujy = dj.y * irj, ujz = dj.z * irj;
-          const T ukx = dk.x * irk, uky = dk.y * irk, ukz = dk.z * irk;
-          const T c = fmin(fmax(ujx * ukx + ujy * uky + ujz * ukz, (T)-1), (T)1);
-          T S0 = 0, S1 = 0;
-#pragma unroll
-          for (int q = 0; q < MQ; ++q) {
-            if (q < nm) {
-              const T P = (T)1 + lam[q] * c;
-              T pw, pm1;
-              if (xint[q] == 0) {
-                pw = (T)1;
-                pm1 = (T)0;
-              } else if (xint[q] > 0) {
-                pm1 = xpow(P, xint[q] - 1, (T)0);
-                pw = pm1 * P;
-              } else {
-                pw = fpow(P, xi[q]);
-                pm1 = fpow(P, xi[q] - (T)1);
-              }
-              S0 += A[q] * pw;
-              S1 += A[q] * xi[q] * lam[q] * pm1;
-            }
-          }
-          const T B = Ej * Ek * H;
-          const T cj = w * S1 * B * irj, ck = w * S1 * B * irk;
-          const T aj = w * S0 * Epj * Ek * H, ak = w * S0 * Ej * Epk * H;
-          T gjx = cj * (ukx - c * ujx) + aj * ujx;
-          T gjy = cj * (uky - c * ujy) + aj * ujy;
-          T gjz = cj * (ukz - c * ujz) + aj * ujz;
-          T gkx = ck * (ujx - c * ukx) + ak * ukx;
-          T gky = ck * (ujy - c * uky) + ak * uky;
-          T gkz = ck * (ujz - c * ukz) + ak * ukz;
-          if (g4) {
-            const T h = w * S0 * Ej * Ek * Hp / rjk;
-            gjx += h * rx;
-            gjy += h * ry;
-            gjz += h * rz;
-            gkx -= h * rx;
-            gky -= h * ry;
-            gkz -= h * rz;
-          }
-          atomicAdd(&sl.gs[3 * ej + 0], gjx);
-          atomicAdd(&sl.gs[3 * ej + 1], gjy);
-          atomicAdd(&sl.gs[3 * ej + 2], gjz);
-          atomicAdd(&sl.gs[3 * ek + 0], gkx);
-          atomicAdd(&sl.gs[3 * ek + 1], gky);
-          atomicAdd(&sl.gs[3 * ek + 2], gkz);
+          if (g4) pairs_adj_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+          else pairs_adj_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+        } else {
+          if (g4) pairs_adj_p<T, MQ, false, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+          else pairs_adj_p<T, MQ, false, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
         }
       }
       wave_sync();
@@ -585,8 +734,10 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
   const int M = ((b->max_nb + 31) / 32) * 32 + 32;
-  const size_t sb = slice_bytes<T, ADJ>(M, m.nZ);
-  int wpb = 8;
+  const size_t sb = slice_bytes<T, ADJ>(M);
+  // waves per block (water boxes: adjoint 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 /
+  // 2.25 ms with 2 / 4)
+  int wpb = ADJ ? 1 : 2;
   while (wpb > 1 && sb * wpb > 64 * 1024) wpb >>= 1;
   if (sb * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
   AcsfArgs a;
@@ -625,9 +776,10 @@ hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, 
 }  // namespace
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj) {
+  (void)nZ;
   const int M = ((max_nb + 31) / 32) * 32 + 32;
-  if (fp64) return adj ? slice_bytes<double, true>(M, nZ) : slice_bytes<double, false>(M, nZ);
-  return adj ? slice_bytes<float, true>(M, nZ) : slice_bytes<float, false>(M, nZ);
+  if (fp64) return adj ? slice_bytes<double, true>(M) : slice_bytes<double, false>(M);
+  return adj ? slice_bytes<float, true>(M) : slice_bytes<float, false>(M);
 }
 
 hipError_t launch_acsf_fwd(fnet_batch* b, const DevModel& m, long long a0, long long a1, int fp64,

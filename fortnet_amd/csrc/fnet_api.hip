@@ -502,6 +502,7 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
       a.lam[q] = md->g_lambda ? md->g_lambda[g] : 0.0;
       a.xint[q] = (xi >= 0.0 && xi <= 64.0 && std::floor(xi) == xi) ? (int)xi : -1;
       a.scale[q] = std::pow(2.0, 1.0 - xi);
+      a.pdiag[q] = std::pow(1.0 + a.lam[q], xi);
     } else {
       return fail(ctx, FNET_ERR_ARG, "invalid G-function type");
     }

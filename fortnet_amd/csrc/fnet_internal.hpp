@@ -40,6 +40,7 @@ struct AngGroup {
   double xi[kMaxMembers];
   double lam[kMaxMembers];
   double scale[kMaxMembers]; // 2^(1 - xi)
+  double pdiag[kMaxMembers]; // (1 + lam)^xi: the j == k terms, where cos(theta) = 1
 };
 
 // Network shape of every element network.

