@@ -53,6 +53,7 @@ def acsf_flops(batch, geos, ns):
     lists (j == k included, as the reference's double loop) 10 shared by the functions of one
     (type, cutoff, eta, species pair) group, + 8 for the G4 r_jk cutoff, + 6 per function.
     The adjoint pass is counted as 2.5x the forward."""
+    batch.neighbour_list()
     rp, j, _ = batch.neighbours()
     N = len(rp) - 1
     Z = np.concatenate([g.atnum for g in geos])
@@ -81,6 +82,23 @@ def acsf_flops(batch, geos, ns):
         pairs = (count(z1) * count(z2)).sum()
         fl += pairs * (10.0 + (8.0 if t == 4 else 0.0) + 6.0 * nmem)
     return fl
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage on the default workload, as measured with the FETCH_SIZE
+    and WRITE_SIZE counters (scaled by a plain copy of known size) and recorded in
+    profiles/hbm_traffic_water.md; None when the stage has no measurement there."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_water.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 2 and cells[0] == stage:
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(geos, ns, budget_s=20.0):
@@ -173,6 +191,8 @@ def main():
     ap.add_argument("--serve-rates", default="500,2000,5000,20000")
     ap.add_argument("--serve-duration", type=float, default=1.0)
     ap.add_argument("--serve-max-requests", type=int, default=2000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with a plain 512 MiB device copy (x3) to scale HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -207,15 +227,18 @@ def main():
 
     # plain path: FP64, one structure per launch
     b.set_mode(fa.FNET_FP64, fa.FNET_PER_STRUCT)
-    dtp, stagep = time_steps(b, args.plain_steps, 1)
-    plain = args.plain_steps * N / dtp
+    plain, stagep = None, np.zeros(5)
+    if args.plain_steps > 0:
+        dtp, stagep = time_steps(b, args.plain_steps, 1)
+        plain = args.plain_steps * N / dtp
 
     names = ["neighbour", "acsf", "networks", "adjoint", "reduce"]
     k = int(np.argmax(stage))
     ab = algorithmic_bytes(names[k], N, P, nG)
     achieved = ab / (stage[k] * 1e-3) / 1e9
+    traffic = measured_traffic(names[k]) if (args.workload, args.scale) == ("water", 1.0) else None
     roofline = {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "algorithmic_bytes": ab, "kernel_ms": float(stage[k])}
     if names[k] in ("acsf", "adjoint"):
         fl = acsf_flops(b, geos, ns) * (2.5 if names[k] == "adjoint" else 1.0)
@@ -233,7 +256,7 @@ def main():
                    "atoms": N, "pairs": P, "n_features": nG,
                    "precision": "fp32 ACSF/networks/forces, fp64 energy reduction"},
         "plain_fp64_per_structure_atoms_per_s": plain,
-        "batched_over_plain": value / plain,
+        "batched_over_plain": value / plain if plain else None,
         "fp32_vs_fp64": {"max_energy_err_ev_per_atom": e_err, "force_rmse_ev_per_A": f_rmse},
         "stages_ms": dict(zip(names, [float(x) for x in stage])),
         "plain_stages_ms": dict(zip(names, [float(x) for x in stagep])),
@@ -247,6 +270,12 @@ def main():
                                        set(rates[:2]), 300)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(geos, ns, args.cpu_budget)
+    if args.copy_calibration:
+        import ctypes
+        probe = ctypes.CDLL(os.path.join(ROOT, "fortnet_amd", "libfnet_probe.so"))
+        probe.fnet_probe_copy.argtypes = [ctypes.c_size_t, ctypes.c_int]
+        rc = probe.fnet_probe_copy(1 << 29, 3)
+        print("copy calibration: 3 x %d bytes, rc=%d" % (1 << 29, rc), file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 

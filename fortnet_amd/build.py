@@ -1,7 +1,9 @@
 """Builds libfnet_hip.so (HIP, gfx950) in-tree with hipcc.
 
 ``python -m fortnet_amd.build`` compiles every ``csrc/*.hip`` translation unit in parallel and
-links the shared library next to this file, where ``fortnet_amd._lib`` loads it.
+links the shared library next to this file, where ``fortnet_amd._lib`` loads it; it also links
+``libfnet_probe.so`` (``probe/``: a plain copy kernel that bench.py uses to scale the HBM byte
+counters when it runs under a profiler).
 """
 import concurrent.futures
 import os
@@ -12,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfnet_hip.so")
+PROBE_LIB = os.path.join(HERE, "libfnet_probe.so")  # copy kernel for counter calibration
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FNET_OFFLOAD_ARCH", "gfx950")
 
@@ -36,21 +39,27 @@ def _compile(src):
     return obj, out
 
 
+def _link(objs, lib):
+    tmp = lib + ".tmp%d" % os.getpid()
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout)
+    os.replace(tmp, lib)
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(_compile, srcs))
+    probe = os.path.join(HERE, "probe", "fnet_copy_probe.hip")
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs) + 1)) as ex:
+        results = list(ex.map(_compile, srcs + [probe]))
     if verbose:
         for _, out in results:
             if out.strip():
                 print(out)
-    tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("link failed:\n" + r.stdout)
-    os.replace(tmp, LIB)
+    _link([o for o, _ in results[:-1]], LIB)
+    _link([results[-1][0]], PROBE_LIB)
     return LIB
 
 

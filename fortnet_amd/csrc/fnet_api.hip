@@ -159,7 +159,6 @@ int do_neighbours(fnet_batch* b) {
   const double rc = m.rcmax;
   HIPCHECK(ctx, launch_struct_prep(b, rc, s));
   HIPCHECK(ctx, launch_cell_bin(b, s));
-  HIPCHECK(ctx, hipMemsetAsync(b->d_maxnb, 0, sizeof(int), s));
   if (b->launch == FNET_PER_STRUCT) {
     for (int st = 0; st < b->S; ++st) {
       HIPCHECK(ctx, launch_neigh_count(b, rc, b->h_off[st], b->h_off[st + 1], s));
@@ -169,7 +168,7 @@ int do_neighbours(fnet_batch* b) {
     HIPCHECK(ctx, launch_neigh_count(b, rc, 0, b->N, s));
     HIPCHECK(ctx, launch_cells_count(b, rc, 0, b->N, s));
   }
-  HIPCHECK(ctx, launch_scan(b->cnt, b->rowptr, b->N, b->scan_tmp, s));
+  HIPCHECK(ctx, launch_scan(b->cnt, b->rowptr, b->N, b->scan_tmp, s, b->d_maxnb));
   int hv[2] = {0, 0};
   HIPCHECK(ctx, hipMemcpyAsync(&hv[0], b->rowptr + b->N, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(ctx, hipMemcpyAsync(&hv[1], b->d_maxnb, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -79,7 +79,6 @@ struct CellArgs {
   long long a0, a1;
   int K;  // sort capacity per wave (power of two >= largest row)
   int* cnt;
-  int* maxnb;
   const int* rowptr;
   int* nj;
   int* nimg;
@@ -198,10 +197,7 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
         }
       }
   if (!WRITE) {
-    if (lane == 0) {
-      a.cnt[i] = count;
-      atomicMax(a.maxnb, count);
-    }
+    if (lane == 0) a.cnt[i] = count;
     return;
   }
   const int m = count < K ? count : K;
@@ -262,7 +258,6 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
   while (K < b->max_nb) K <<= 1;
   a.K = K;
   a.cnt = b->cnt;
-  a.maxnb = b->d_maxnb;
   a.rowptr = b->rowptr;
   a.nj = b->nj;
   a.nimg = b->nimg;
@@ -289,7 +284,7 @@ hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s) {
   k_cell_bin<<<grid, 256, 0, s>>>(b->xyz, b->atom_struct, b->sinfo, b->cellinfo, b->cell_off, 0,
                                   b->N, b->atom_cell, b->atom_m, b->cell_count);
   FNET_HIP_TRY(hipGetLastError());
-  FNET_HIP_TRY(launch_scan(b->cell_count, b->cell_start, b->ncells, b->cell_scan_tmp, s));
+  FNET_HIP_TRY(launch_scan(b->cell_count, b->cell_start, b->ncells, b->cell_scan_tmp, s, nullptr));
   FNET_HIP_TRY(hipMemsetAsync(b->cell_count, 0, sizeof(int) * b->ncells, s));
   k_cell_scatter<<<grid, 256, 0, s>>>(b->atom_struct, b->cellinfo, 0, b->N, b->atom_cell,
                                       b->cell_start, b->cell_count, b->cell_atoms);

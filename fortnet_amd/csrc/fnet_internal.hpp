@@ -183,7 +183,8 @@ hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s);
 hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
                              hipStream_t s);
-hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s);
+hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s,
+                       int* maxout);
 long long scan_tmp_size(long long n);
 
 // launchers (fnet_cells.hip)

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256)
     k_neigh(const double* __restrict__ xyz, const long long* __restrict__ off,
             const int* __restrict__ atom_struct, const StructInfo* __restrict__ sinfo,
             const double* __restrict__ lat, double rc, long long a0, long long a1,
-            int* __restrict__ cnt, int* __restrict__ maxnb, const int* __restrict__ rowptr,
+            int* __restrict__ cnt, const int* __restrict__ rowptr,
             int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd,
             const int4* __restrict__ cellinfo) {
 #pragma clang fp contract(off)
@@ -223,10 +223,7 @@ __global__ void __launch_bounds__(256)
           }
         }
   }
-  if (!WRITE && lane == 0) {
-    cnt[i] = count;
-    atomicMax(maxnb, count);
-  }
+  if (!WRITE && lane == 0) cnt[i] = count;
 }
 
 // ---- exclusive scan of the per-atom counts -------------------------------------------------
@@ -234,12 +231,25 @@ constexpr int kScanBlock = 1024;
 
 __global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict__ in,
                                                           int* __restrict__ out, long long n,
-                                                          int* __restrict__ totals) {
+                                                          int* __restrict__ totals,
+                                                          int* __restrict__ bmax) {
   __shared__ int buf[kScanBlock];
+  __shared__ int wmax[kScanBlock / 64];
   const long long g = (long long)blockIdx.x * kScanBlock + threadIdx.x;
   const int v = (g < n) ? in[g] : 0;
+  if (bmax) {
+    int m = v;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  }
   buf[threadIdx.x] = v;
   __syncthreads();
+  if (bmax && threadIdx.x == 0) {
+    int m = wmax[0];
+    for (int w = 1; w < kScanBlock / 64; ++w) m = max(m, wmax[w]);
+    bmax[blockIdx.x] = m;
+  }
   for (int st = 1; st < kScanBlock; st <<= 1) {
     int add = (threadIdx.x >= st) ? buf[threadIdx.x - st] : 0;
     __syncthreads();
@@ -251,10 +261,17 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict
 }
 
 __global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* __restrict__ totals,
-                                                           long long nb, int* __restrict__ last) {
+                                                           long long nb, int* __restrict__ last,
+                                                           const int* __restrict__ bmax,
+                                                           int* __restrict__ maxout) {
   __shared__ int buf[kScanBlock];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
+  if (maxout && threadIdx.x == 0) {
+    int m = 0;
+    for (long long b = 0; b < nb; ++b) m = max(m, bmax[b]);
+    *maxout = m;
+  }
   __syncthreads();
   for (long long b0 = 0; b0 < nb; b0 += kScanBlock) {
     const long long g = b0 + threadIdx.x;
@@ -282,15 +299,22 @@ __global__ void k_scan_add(int* __restrict__ out, long long n, const int* __rest
 
 }  // namespace
 
-long long scan_tmp_size(long long n) { return (n + kScanBlock - 1) / kScanBlock + 1; }
+// scratch of launch_scan: block totals (nb + 1) and block maxima (nb)
+long long scan_tmp_size(long long n) { return 2 * ((n + kScanBlock - 1) / kScanBlock) + 1; }
 
-hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s) {
+// Exclusive scan of cnt[0..n) into rowptr[0..n] (rowptr[n] = total); with maxout, also the
+// largest count (the reduction rides on the scan's own pass over the counts).
+hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s,
+                       int* maxout) {
   const long long nb = (n + kScanBlock - 1) / kScanBlock;
   if (nb == 0) {
-    return hipMemsetAsync(rowptr, 0, sizeof(int), s);
+    FNET_HIP_TRY(hipMemsetAsync(rowptr, 0, sizeof(int), s));
+    if (maxout) FNET_HIP_TRY(hipMemsetAsync(maxout, 0, sizeof(int), s));
+    return hipSuccess;
   }
-  k_scan_block<<<(unsigned)nb, kScanBlock, 0, s>>>(cnt, rowptr, n, tmp);
-  k_scan_totals<<<1, kScanBlock, 0, s>>>(tmp, nb, rowptr + n);
+  int* bmax = maxout ? tmp + nb + 1 : nullptr;
+  k_scan_block<<<(unsigned)nb, kScanBlock, 0, s>>>(cnt, rowptr, n, tmp, bmax);
+  k_scan_totals<<<1, kScanBlock, 0, s>>>(tmp, nb, rowptr + n, bmax, maxout);
   k_scan_add<<<(unsigned)nb, kScanBlock, 0, s>>>(rowptr, n, tmp);
   return hipGetLastError();
 }
@@ -307,8 +331,8 @@ hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long 
   if (n <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((n + 3) / 4);
   k_neigh<false, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
-                                               rc, a0, a1, b->cnt, b->d_maxnb, nullptr, nullptr,
-                                               nullptr, nullptr, b->cellinfo);
+                                               rc, a0, a1, b->cnt, nullptr, nullptr, nullptr,
+                                               nullptr, b->cellinfo);
   return hipGetLastError();
 }
 
@@ -319,12 +343,12 @@ hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a
   const unsigned grid = (unsigned)((n + 3) / 4);
   if (fp64) {
     k_neigh<true, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
-                                                rc, a0, a1, nullptr, nullptr, b->rowptr, b->nj,
-                                                b->nimg, (double4*)b->nd, b->cellinfo);
+                                                rc, a0, a1, nullptr, b->rowptr, b->nj, b->nimg,
+                                                (double4*)b->nd, b->cellinfo);
   } else {
     k_neigh<true, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
-                                               rc, a0, a1, nullptr, nullptr, b->rowptr, b->nj,
-                                               b->nimg, (float4*)b->nd, b->cellinfo);
+                                               rc, a0, a1, nullptr, b->rowptr, b->nj, b->nimg,
+                                               (float4*)b->nd, b->cellinfo);
   }
   return hipGetLastError();
 }
