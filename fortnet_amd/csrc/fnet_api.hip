@@ -57,6 +57,8 @@ void free_model(DevModel& m) {
   dfree(m.ext_in);
   dfree(m.params);
   dfree(m.params_f);
+  dfree(m.wpack);
+  m.wpack = nullptr;
   m.radial = nullptr;
   m.groups = nullptr;
   m.zmean = m.zinv = nullptr;
@@ -76,16 +78,34 @@ void invalidate(fnet_batch* b) {
   b->have_nl = b->have_G = b->have_E = b->have_F = false;
 }
 
+// Packs the host copy of the parameters into MFMA fragments and uploads them (FP32 networks on
+// the matrix cores); a model whose shape the MFMA kernel does not take keeps wpack empty.
+int upload_wpack(fnet_ctx* ctx, DevModel& m, hipStream_t s) {
+  if (!mlp_mfma_waves(m.shape)) return FNET_OK;
+  std::vector<float> pk;
+  pack_mfma_weights(m.shape, m.nspecies, m.h_params.data(), pk, m.pk_stride, m.pk_fw, m.pk_rw,
+                    m.pk_b);
+  if (!m.wpack) HIPCHECK(ctx, dalloc(&m.wpack, pk.size()));
+  HIPCHECK(ctx, hipMemcpyAsync(m.wpack, pk.data(), sizeof(float) * pk.size(),
+                               hipMemcpyHostToDevice, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  return FNET_OK;
+}
+
 // (Re)builds the tiles of the network kernels for the batch's launch mode and precision.
 int ensure_tiles(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
   const int TA = mlp_tile_atoms(m.shape, b->precision != FNET_FP32);
-  if (b->tile_atoms == TA && b->tile_launch == b->launch && b->tile_sp) return FNET_OK;
-  std::vector<int> perm, tsp, tstart, tcnt;
+  const int MW = mlp_mfma_waves(m.shape);  // 16 atoms per wave on the MFMA kernel
+  if (b->tile_atoms == TA && b->mtile_waves == MW && b->tile_launch == b->launch && b->tile_sp)
+    return FNET_OK;
+  std::vector<int> perm, tsp, tstart, tcnt, msp, mstart, mcnt;
   perm.reserve(b->N);
   b->h_tile_first.assign(b->S + 1, 0);
+  b->h_mtile_first.assign(b->S + 1, 0);
   auto add_tiles = [&](int sp, const std::vector<int>& atoms) {
+    const int p0 = (int)perm.size();
     for (size_t k = 0; k < atoms.size(); k += TA) {
       tsp.push_back(sp);
       tstart.push_back((int)perm.size());
@@ -93,10 +113,17 @@ int ensure_tiles(fnet_batch* b) {
       tcnt.push_back(c);
       for (int q = 0; q < c; ++q) perm.push_back(atoms[k + q]);
     }
+    if (MW)
+      for (size_t k = 0; k < atoms.size(); k += 16 * MW) {
+        msp.push_back(sp);
+        mstart.push_back(p0 + (int)k);
+        mcnt.push_back((int)std::min<size_t>(16 * MW, atoms.size() - k));
+      }
   };
   if (b->launch == FNET_PER_STRUCT) {
     for (int s = 0; s < b->S; ++s) {
       b->h_tile_first[s] = (int)tsp.size();
+      b->h_mtile_first[s] = (int)msp.size();
       for (int sp = 0; sp < m.nspecies; ++sp) {
         std::vector<int> atoms;
         for (long long i = b->h_off[s]; i < b->h_off[s + 1]; ++i)
@@ -105,6 +132,7 @@ int ensure_tiles(fnet_batch* b) {
       }
     }
     b->h_tile_first[b->S] = (int)tsp.size();
+    b->h_mtile_first[b->S] = (int)msp.size();
   } else {
     std::vector<std::vector<int>> by(m.nspecies);
     for (long long i = 0; i < b->N; ++i) by[b->h_species[i]].push_back((int)i);
@@ -114,13 +142,24 @@ int ensure_tiles(fnet_batch* b) {
   dfree(b->tile_sp);
   dfree(b->tile_start);
   dfree(b->tile_cnt);
+  dfree(b->mtile_sp);
+  dfree(b->mtile_start);
+  dfree(b->mtile_cnt);
   b->perm = b->tile_sp = b->tile_start = b->tile_cnt = nullptr;
+  b->mtile_sp = b->mtile_start = b->mtile_cnt = nullptr;
   hipStream_t s = ctx->stream;
   HIPCHECK(ctx, upload(&b->perm, perm.data(), perm.size(), s));
   HIPCHECK(ctx, upload(&b->tile_sp, tsp.data(), tsp.size(), s));
   HIPCHECK(ctx, upload(&b->tile_start, tstart.data(), tstart.size(), s));
   HIPCHECK(ctx, upload(&b->tile_cnt, tcnt.data(), tcnt.size(), s));
+  if (MW) {
+    HIPCHECK(ctx, upload(&b->mtile_sp, msp.data(), msp.size(), s));
+    HIPCHECK(ctx, upload(&b->mtile_start, mstart.data(), mstart.size(), s));
+    HIPCHECK(ctx, upload(&b->mtile_cnt, mcnt.data(), mcnt.size(), s));
+  }
   HIPCHECK(ctx, hipStreamSynchronize(s));
+  b->nmtiles = (int)msp.size();
+  b->mtile_waves = MW;
   b->ntiles = (int)tsp.size();
   b->tile_atoms = TA;
   b->tile_launch = b->launch;
@@ -230,11 +269,19 @@ int do_mlp(fnet_batch* b, int gradin) {
   const int fp64 = b->precision != FNET_FP32;
   int rc = ensure_tiles(b);
   if (rc) return rc;
+  const bool mfma = !fp64 && m.wpack && b->mtile_waves > 0;
   if (b->launch == FNET_PER_STRUCT) {
     for (int st = 0; st < b->S; ++st) {
-      const int t0 = b->h_tile_first[st], t1 = b->h_tile_first[st + 1];
-      HIPCHECK(ctx, launch_mlp_fwd(b, m, t0, t1 - t0, fp64, gradin, ctx->stream));
+      if (mfma) {
+        const int t0 = b->h_mtile_first[st], t1 = b->h_mtile_first[st + 1];
+        HIPCHECK(ctx, launch_mlp_mfma(b, m, t0, t1 - t0, gradin, ctx->stream));
+      } else {
+        const int t0 = b->h_tile_first[st], t1 = b->h_tile_first[st + 1];
+        HIPCHECK(ctx, launch_mlp_fwd(b, m, t0, t1 - t0, fp64, gradin, ctx->stream));
+      }
     }
+  } else if (mfma) {
+    HIPCHECK(ctx, launch_mlp_mfma(b, m, 0, b->nmtiles, gradin, ctx->stream));
   } else {
     HIPCHECK(ctx, launch_mlp_fwd(b, m, 0, b->ntiles, fp64, gradin, ctx->stream));
   }
@@ -560,6 +607,13 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   HIPCHECK(ctx, dalloc(&m.params_f, np));
   HIPCHECK(ctx, launch_to_float(m.params, m.params_f, (long long)np, s));
   HIPCHECK(ctx, hipStreamSynchronize(s));
+  {
+    const int prc = upload_wpack(ctx, m, s);
+    if (prc) {
+      free_model(m);
+      return prc;
+    }
+  }
   m.valid = true;
   free_model(ctx->model);
   ctx->model = m;
@@ -579,7 +633,7 @@ int fnet_params_upload(fnet_ctx* ctx, const double* params) {
                                ctx->stream));
   HIPCHECK(ctx, launch_to_float(m.params, m.params_f, (long long)np, ctx->stream));
   HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-  return FNET_OK;
+  return upload_wpack(ctx, m, ctx->stream);
 }
 
 }  // extern "C"
@@ -760,7 +814,7 @@ void fnet_batch_destroy(fnet_batch* b) {
                   b->rowptr, b->scan_tmp, b->nj, b->nimg, b->nd, b->d_maxnb, b->G, b->cG,
                   b->Eatom, b->Estruct, b->F, b->lossgrad, b->d_targets, b->cellinfo,
                   b->cell_off, b->cell_count, b->cell_start, b->cell_atoms, b->atom_cell,
-                  b->atom_m, b->cell_scan_tmp};
+                  b->atom_m, b->cell_scan_tmp, b->mtile_sp, b->mtile_start, b->mtile_cnt};
   for (void* p : ptrs) dfree(p);
   if (b->ctx->last_train_batch == b) b->ctx->last_train_batch = nullptr;
   delete b;
@@ -1020,6 +1074,10 @@ int fnet_optimizer_step_sd(fnet_ctx* ctx, double lr, double max_disp, double tol
   HIPCHECK(ctx, hipMemcpyAsync(gm.data(), gmax, sizeof(double) * m.nspecies,
                                hipMemcpyDeviceToHost, ctx->stream));
   HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  {
+    const int prc = upload_wpack(ctx, m, ctx->stream);
+    if (prc) return prc;
+  }
   if (params_out) std::memcpy(params_out, m.h_params.data(), sizeof(double) * np);
   if (converged) {
     int c = 1;

@@ -79,6 +79,12 @@ struct DevModel {
   int* ext_in = nullptr;        // [next_in] 0-based ext columns
   double* params = nullptr;     // [nspecies][nparams]
   float* params_f = nullptr;    // FP32 copy
+  // FP32 weights packed as MFMA operand fragments (per species, stride pk_stride floats):
+  // forward fragments of every layer at pk_fw[l], reverse (transposed) fragments at pk_rw[l],
+  // biases zero-padded to a multiple of 16 at pk_b[l]
+  float* wpack = nullptr;
+  long long pk_stride = 0;
+  long long pk_fw[kMaxLayers] = {}, pk_rw[kMaxLayers] = {}, pk_b[kMaxLayers] = {};
   std::vector<double> h_params;
   std::vector<int> h_species_z;
   bool valid = false;
@@ -138,6 +144,13 @@ struct fnet_batch {
   int tile_atoms = 0;
   int tile_launch = -1;          // launch mode the tiles were built for
   std::vector<int> h_tile_first; // [S+1] tiles of each structure (per-structure launches)
+  // block tiles of the MFMA network kernel (16 atoms per wave, mtile_waves waves per block)
+  int* mtile_sp = nullptr;
+  int* mtile_start = nullptr;
+  int* mtile_cnt = nullptr;
+  int nmtiles = 0;
+  int mtile_waves = 0;
+  std::vector<int> h_mtile_first; // [S+1]
   // neighbour list (CSR)
   int* cnt = nullptr;          // [N]
   int* rowptr = nullptr;       // [N+1]
@@ -215,6 +228,14 @@ hipError_t launch_sd_step(double* params, const double* grad, long long n, doubl
                           double max_disp, double tol, double* scratch, int nspecies,
                           hipStream_t s);
 int mlp_tile_atoms(const NetShape& sh, int fp64);
+
+// launchers (fnet_mlp_mfma.hip): FP32 element networks on MFMA 16x16x4 f32
+int mlp_mfma_waves(const NetShape& sh);  // waves per block, 0 if the shape is not supported
+void pack_mfma_weights(const NetShape& sh, int nspecies, const double* params,
+                       std::vector<float>& out, long long& stride, long long* fw, long long* rw,
+                       long long* pb);
+hipError_t launch_mlp_mfma(fnet_batch* b, const DevModel& m, int first_tile, int ntiles,
+                           int want_grad_in, hipStream_t s);
 hipError_t launch_to_float(const double* in, float* out, long long n, hipStream_t s);
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj);

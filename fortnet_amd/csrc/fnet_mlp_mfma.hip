@@ -1,0 +1,363 @@
+// FP32 element networks on the matrix cores of gfx950 (v_mfma_f32_16x16x4_f32).
+//
+// Same mathematics as the generic network kernel (fprop, network.F90:146-180 / iPredict :314-347;
+// the input gradient of fdevi, network.F90:183-244, in reverse mode, one sweep per output), laid
+// out for MFMA: each wave owns 16 atoms of one element and runs them through every layer as
+//   Z^T [d_out x 16] = W^T [d_out x d_in] . X^T [d_in x 16]
+// in 16 x 16 output blocks with K = 4 per instruction.  W^T fragments come from a copy of the
+// FP32 weights packed at model upload so that each instruction's A operand is one coalesced
+// 256-byte load (L1/L2-resident, shared by all waves); X^T fragments come from the wave's
+// activation rows in LDS, 4 K-steps per ds_read_b128 (the K order inside a 16-wide chunk is
+// permuted to k = 16 c + 4 g + s, identically in the packed weights, so lane group g reads four
+// consecutive features).  The accumulator fragment (rows = outputs 4 g + r, column = atom) is
+// activated in registers, f'(z) of every hidden layer is kept in LDS in fragment order, and the
+// activations go back to LDS as the next layer's rows.  The reverse sweep runs the transposed
+// products delta_l^T = W_l . delta_{l+1}^T on separately packed fragments.  Products and sums are
+// FP32 (as the generic FP32 kernel); per-atom outputs and input gradients are written in FP64.
+#include <algorithm>
+
+#include "fnet_internal.hpp"
+
+namespace fnet {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// every layer width is padded to a multiple of 32: output blocks are computed in pairs
+__host__ __device__ __forceinline__ int r32(int d) { return (d + 31) & ~31; }
+
+__device__ __forceinline__ float actf(int id, float x) {
+  switch (id) {
+    case 7: return expf(-(x * x));                                // gaussian
+    case 4: return logf(1.0f + expf(x));                          // softplus
+    case 5: return (sqrtf(x * x + 1.0f) - 1.0f) / 2.0f + x;       // bent identity
+    case 2: return x > 0.0f ? x : 0.0f;                           // relu
+    case 3: return (0.01f * x > x) ? 0.01f * x : x;               // leaky relu
+    case 1: return 1.0f / (1.0f + expf(-x));                      // sigmoid
+    case 8: return x > 0.0f ? 1.0f : 0.0f;                        // heaviside
+    case 6: return atanf(x);                                      // atan
+    case 9: return x;                                             // linear
+    default: return tanhf(x);                                     // tanh
+  }
+}
+
+__device__ __forceinline__ float actd(int id, float x) {
+  switch (id) {
+    case 7: return -2.0f * x * expf(-(x * x));
+    case 4: return 1.0f / (1.0f + expf(-x));
+    case 5: return x / (2.0f * sqrtf(x * x + 1.0f)) + 1.0f;
+    case 2: return x >= 0.0f ? 1.0f : 0.0f;
+    case 3: return x >= 0.0f ? 1.0f : 0.01f;
+    case 1: {
+      const float s = 1.0f / (1.0f + expf(-x));
+      return s * (1.0f - s);
+    }
+    case 8: return 0.0f;
+    case 6: return 1.0f / (x * x + 1.0f);
+    case 9: return 1.0f;
+    default: {
+      const float t = tanhf(x);
+      return 1.0f - t * t;
+    }
+  }
+}
+
+struct MfmaArgs {
+  NetShape sh;
+  const float* wpack;
+  long long pk_stride;
+  long long fw[kMaxLayers], rw[kMaxLayers], pb[kMaxLayers];
+  int fpoff[kMaxLayers];  // offset (floats) of f'(z) of hidden activation layer m in the wave's store
+  const float* params_f;  // [nspecies][nparams] FP32 (last-layer columns of the reverse sweep)
+  const double* G;        // [N][nG]
+  const double* ext;
+  int next;
+  const int* ext_in;
+  int nG, nout;
+  const double* zmean;
+  const double* zinv;
+  const int* perm;
+  const int* tile_sp;
+  const int* tile_start;
+  const int* tile_cnt;
+  int first_tile;
+  int ldx;        // row stride (floats) of the wave's activation rows
+  int wave_floats;
+  double* Eatom;  // [N][nout]
+  double* cG;     // [N][nout][nG]
+};
+
+// Row stride and per-wave LDS (floats) for a network shape: 16 activation rows of the widest
+// layer (+4 floats so that rows start on different banks) and the fragment slots of every hidden
+// layer, which hold z, then f'(z), then the reverse-sweep delta of that layer.
+void wave_lds(const NetShape& sh, int& ldx, int& wave_floats, int* fpoff) {
+  int dmax = 0;
+  for (int l = 0; l < sh.nl; ++l) dmax = std::max(dmax, r32(sh.dims[l]));
+  ldx = dmax + 4;
+  int fp = 0;
+  for (int m = 0; m < kMaxLayers; ++m) fpoff[m] = 0;
+  for (int m = 1; m < sh.nl - 1; ++m) {
+    fpoff[m] = fp;
+    fp += r32(sh.dims[m]) * 16;
+  }
+  wave_floats = 16 * ldx + fp;
+}
+
+// Two 16 x 16 output blocks (ob0, ob0 + 1) of Z^T = A . X^T over Kc 16-wide K chunks: A fragments
+// from the packed operand Af (block stride Kc * 256 floats), X^T fragments from the wave's rows.
+__device__ __forceinline__ void gemm_pair(const float* __restrict__ Af, int ob0, int Kc,
+                                          const float* X, int ldx, int lane, int j, int g,
+                                          f4& c0, f4& c1) {
+  const float* A0 = Af + (long long)ob0 * Kc * 256 + lane;
+  const float* A1 = A0 + (long long)Kc * 256;
+#pragma unroll 1
+  for (int c = 0; c < Kc; ++c) {
+    const f4 xb = *(const f4*)(X + j * ldx + 16 * c + 4 * g);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0[(c * 4 + s) * 64], xb[s], c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1[(c * 4 + s) * 64], xb[s], c1, 0, 0, 0);
+    }
+  }
+}
+
+template <bool GRADIN>
+__global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
+  extern __shared__ __align__(16) float smf[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int tile = a.first_tile + blockIdx.x;
+  const int sp = a.tile_sp[tile], start = a.tile_start[tile], cnt = a.tile_cnt[tile];
+  const int a0 = 16 * w;
+  if (a0 >= cnt) return;  // whole wave
+  const int nv = min(16, cnt - a0);
+  const int nl = a.sh.nl;
+  const int act = a.sh.act[sp];
+  const int ldx = a.ldx;
+  float* X = smf + (size_t)w * a.wave_floats;
+  float* FP = X + 16 * ldx;
+  const float* WP = a.wpack + (long long)sp * a.pk_stride;
+  const bool valid = j < nv;
+  const long long atom = valid ? (long long)a.perm[start + a0 + j] : 0;
+
+  // input rows: standardised features (and external features), zero beyond d0 and nv
+  const int d0 = a.sh.dims[0], K0 = r32(d0);
+  for (int idx = lane; idx < 16 * K0; idx += 64) {
+    const int r = idx / K0, k = idx - r * K0;
+    float v = 0.0f;
+    if (r < nv && k < d0) {
+      const long long at = a.perm[start + a0 + r];
+      v = k < a.nG ? (float)((a.G[at * a.nG + k] - a.zmean[k]) * a.zinv[k])
+                   : (float)a.ext[at * a.next + a.ext_in[k - a.nG]];
+    }
+    X[r * ldx + k] = v;
+  }
+  wave_sync();
+
+  for (int l = 0; l < nl - 1; ++l) {
+    const int Kc = r32(a.sh.dims[l]) >> 4, Nb = r32(a.sh.dims[l + 1]) >> 4;
+    const float* Wf = WP + a.fw[l];
+    const float* Bp = WP + a.pb[l];
+    const bool hidden = l < nl - 2;
+    float* zs = FP + a.fpoff[l + 1];
+    for (int ob0 = 0; ob0 < Nb; ob0 += 2) {
+      f4 c0 = *(const f4*)(Bp + 16 * ob0 + 4 * g);
+      f4 c1 = *(const f4*)(Bp + 16 * ob0 + 16 + 4 * g);
+      gemm_pair(Wf, ob0, Kc, X, ldx, lane, j, g, c0, c1);
+      if (hidden) {
+        *(f4*)(zs + (ob0 * 64 + lane) * 4) = c0;
+        *(f4*)(zs + ((ob0 + 1) * 64 + lane) * 4) = c1;
+      } else if (valid) {
+        // linear output layer (nout <= 16): lane group g holds outputs 4 g + r of atom j
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int o = 4 * g + r;
+          if (o < a.nout) a.Eatom[atom * a.nout + o] = (double)c0[r];
+        }
+      }
+    }
+    wave_sync();  // every lane has read this layer's rows before they are overwritten
+    if (hidden) {
+      // activation: z in the fragment slots -> next rows y = f(z), slots <- f'(z)
+#pragma unroll 1
+      for (int q = 0; q < 4 * Nb; ++q) {
+        const int ob = q >> 2, r = q & 3;
+        float* slot = zs + (ob * 64 + lane) * 4 + r;
+        const float z = *slot;
+        X[j * ldx + 16 * ob + 4 * g + r] = actf(act, z);
+        *slot = actd(act, z);
+      }
+      wave_sync();
+    }
+  }
+  if (!GRADIN) return;
+
+  // reverse sweep of the single output (input gradient = row of the Jacobian)
+  double* out = a.cG + atom * a.nG;
+  int m = nl - 2;
+  {
+    // delta of activation layer m from the one-hot output: column 0 of ww(m), times f'(z_m)
+    const int dm = a.sh.dims[m], Kb = r32(dm) >> 4;
+    const float* col = a.params_f + (long long)sp * a.sh.nparams + a.sh.woff[m];
+    if (m >= 1) {
+      float* sl = FP + a.fpoff[m];
+#pragma unroll 1
+      for (int q = 0; q < 4 * Kb; ++q) {
+        const int ib = q >> 2, r = q & 3, k = 16 * ib + 4 * g + r;
+        float* slot = sl + (ib * 64 + lane) * 4 + r;
+        const float d = (k < dm ? col[k] : 0.0f) * *slot;
+        *slot = d;
+        X[j * ldx + k] = d;
+      }
+      wave_sync();
+    } else if (valid) {
+      for (int k = 4 * g; k < a.nG; k += 16)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (k + r < a.nG) out[k + r] = (double)col[k + r] * a.zinv[k + r];
+    }
+  }
+  // delta_{m-1}^T = W_{m-1} . delta_m^T (rows in X), times f'(z_{m-1}) for hidden layers
+  for (; m >= 1; --m) {
+    const int Mb = r32(a.sh.dims[m - 1]) >> 4, Kc = r32(a.sh.dims[m]) >> 4;
+    const float* Rf = WP + a.rw[m - 1];
+    float* sl = FP + a.fpoff[m - 1];
+    for (int ib0 = 0; ib0 < Mb; ib0 += 2) {
+      f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+      gemm_pair(Rf, ib0, Kc, X, ldx, lane, j, g, c0, c1);
+      if (m - 1 >= 1) {
+        f4* s0 = (f4*)(sl + (ib0 * 64 + lane) * 4);
+        f4* s1 = (f4*)(sl + ((ib0 + 1) * 64 + lane) * 4);
+        *s0 = c0 * *s0;
+        *s1 = c1 * *s1;
+      } else if (valid) {
+        // input layer: the ACSF part of d out / d x feeds the force adjoint (FP64, z-score factor)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int f0 = 16 * ib0 + 4 * g + r, f1 = f0 + 16;
+          if (f0 < a.nG) out[f0] = (double)c0[r] * a.zinv[f0];
+          if (f1 < a.nG) out[f1] = (double)c1[r] * a.zinv[f1];
+        }
+      }
+    }
+    wave_sync();
+    if (m - 1 >= 1) {
+      // delta_{m-1} fragments -> rows for the next product
+#pragma unroll 1
+      for (int q = 0; q < 4 * Mb; ++q) {
+        const int ib = q >> 2, r = q & 3;
+        X[j * ldx + 16 * ib + 4 * g + r] = sl[(ib * 64 + lane) * 4 + r];
+      }
+      wave_sync();
+    }
+  }
+}
+
+}  // namespace
+
+int mlp_mfma_waves(const NetShape& sh) {
+  // single-output networks (the energy), layer widths up to kMaxDim
+  if (sh.nl < 2 || sh.dims[sh.nl - 1] != 1) return 0;
+  for (int l = 0; l < sh.nl; ++l)
+    if (sh.dims[l] < 1 || sh.dims[l] > kMaxDim) return 0;
+  int ldx, wf, fpoff[kMaxLayers];
+  wave_lds(sh, ldx, wf, fpoff);
+  int waves = 4;
+  while (waves > 1 && (size_t)waves * wf * sizeof(float) > 64 * 1024) waves >>= 1;
+  if ((size_t)waves * wf * sizeof(float) > 64 * 1024) return 0;
+  return waves;
+}
+
+// Packs the FP32 weights of every species into MFMA A-operand fragments (see the header comment),
+// every layer width padded to a multiple of 32.  Forward fragment of layer l: index
+// ((ob * Kc + c) * 4 + s) * 64 + lane, lane = 16 g + i, holds ww_l(k = 16 c + 4 g + s,
+// o = 16 ob + i); reverse fragment: ((ib * Oc + c) * 4 + s) * 64 + lane holds
+// ww_l(k = 16 ib + i, o = 16 c + 4 g + s); zero outside the layer's dimensions.
+void pack_mfma_weights(const NetShape& sh, int nspecies, const double* params,
+                       std::vector<float>& out, long long& stride, long long* fw, long long* rw,
+                       long long* pb) {
+  long long o = 0;
+  for (int l = 0; l + 1 < sh.nl; ++l) {
+    const long long sz = (long long)r32(sh.dims[l]) * r32(sh.dims[l + 1]);
+    fw[l] = o;
+    o += sz;
+    rw[l] = o;
+    o += sz;
+    pb[l] = o;
+    o += r32(sh.dims[l + 1]);
+  }
+  stride = (o + 63) & ~63LL;
+  out.assign((size_t)stride * nspecies, 0.0f);
+  for (int sp = 0; sp < nspecies; ++sp) {
+    const double* P = params + (long long)sp * sh.nparams;
+    float* D = out.data() + (long long)sp * stride;
+    for (int l = 0; l + 1 < sh.nl; ++l) {
+      const int din = sh.dims[l], dout = sh.dims[l + 1];
+      const int Kc = r32(din) / 16, Oc = r32(dout) / 16;
+      auto ww = [&](int k, int oo) -> float {
+        return (k < din && oo < dout) ? (float)P[sh.woff[l] + (long long)oo * din + k] : 0.0f;
+      };
+      for (int ob = 0; ob < Oc; ++ob)
+        for (int c = 0; c < Kc; ++c)
+          for (int s = 0; s < 4; ++s)
+            for (int lane = 0; lane < 64; ++lane) {
+              const int g = lane / 16, i = lane % 16;
+              D[fw[l] + ((long long)(ob * Kc + c) * 4 + s) * 64 + lane] =
+                  ww(16 * c + 4 * g + s, 16 * ob + i);
+            }
+      for (int ib = 0; ib < Kc; ++ib)
+        for (int c = 0; c < Oc; ++c)
+          for (int s = 0; s < 4; ++s)
+            for (int lane = 0; lane < 64; ++lane) {
+              const int g = lane / 16, i = lane % 16;
+              D[rw[l] + ((long long)(ib * Oc + c) * 4 + s) * 64 + lane] =
+                  ww(16 * ib + i, 16 * c + 4 * g + s);
+            }
+      for (int q = 0; q < dout; ++q) D[pb[l] + q] = (float)P[sh.boff[l + 1] + q];
+    }
+  }
+}
+
+hipError_t launch_mlp_mfma(fnet_batch* b, const DevModel& m, int first_tile, int ntiles,
+                           int want_grad_in, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  MfmaArgs a;
+  a.sh = m.shape;
+  a.wpack = m.wpack;
+  a.pk_stride = m.pk_stride;
+  for (int l = 0; l < kMaxLayers; ++l) {
+    a.fw[l] = m.pk_fw[l];
+    a.rw[l] = m.pk_rw[l];
+    a.pb[l] = m.pk_b[l];
+  }
+  wave_lds(m.shape, a.ldx, a.wave_floats, a.fpoff);
+  a.params_f = m.params_f;
+  a.G = (const double*)b->G;
+  a.ext = b->ext;
+  a.next = b->next;
+  a.ext_in = m.ext_in;
+  a.nG = m.nG;
+  a.nout = m.nout;
+  a.zmean = m.zmean;
+  a.zinv = m.zinv;
+  a.perm = b->perm;
+  a.tile_sp = b->mtile_sp;
+  a.tile_start = b->mtile_start;
+  a.tile_cnt = b->mtile_cnt;
+  a.first_tile = first_tile;
+  a.Eatom = b->Eatom;
+  a.cG = (double*)b->cG;
+  const int waves = b->mtile_waves;
+  const size_t lds = (size_t)waves * a.wave_floats * sizeof(float);
+  if (want_grad_in) k_mlp_mfma<true><<<ntiles, 64 * waves, lds, s>>>(a);
+  else k_mlp_mfma<false><<<ntiles, 64 * waves, lds, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace fnet
