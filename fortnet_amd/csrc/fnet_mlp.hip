@@ -268,22 +268,25 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
   }
 }
 
-// One wave per structure: deterministic FP64 sum of the atomic outputs (predictBatch sums,
-// bpnn.F90:1040-1047; forces.F90:381-385).
+// One block per structure: deterministic FP64 sum of the atomic outputs (predictBatch sums,
+// bpnn.F90:1040-1047; forces.F90:381-385).  Thread t sums atoms t, t + 256, ... in order, then the
+// four wave sums are combined in a fixed order, so the result does not depend on scheduling.
 __global__ void __launch_bounds__(256) k_energy_reduce(const double* __restrict__ Eatom,
                                                       const long long* __restrict__ off, int s0,
-                                                      int s1, int nout,
-                                                      double* __restrict__ Estruct) {
-  const int lane = threadIdx.x & 63;
-  const int s = s0 + (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  if (s >= s1) return;
+                                                      int nout, double* __restrict__ Estruct) {
+  __shared__ double ws[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s = s0 + blockIdx.x;
   const long long a0 = off[s], a1 = off[s + 1];
   for (int t = 0; t < nout; ++t) {
     double v = 0.0;
-    for (long long i = a0 + lane; i < a1; i += 64) v += Eatom[i * nout + t];
+    for (long long i = a0 + threadIdx.x; i < a1; i += 256) v += Eatom[i * nout + t];
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    if (lane == 0) Estruct[(long long)s * nout + t] = v;
+    if (lane == 0) ws[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) Estruct[(long long)s * nout + t] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    __syncthreads();
   }
 }
 
@@ -462,7 +465,7 @@ hipError_t launch_mlp_fwd(fnet_batch* b, const DevModel& m, int first_tile, int 
 hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1, hipStream_t s) {
   const int n = s1 - s0;
   if (n <= 0) return hipSuccess;
-  k_energy_reduce<<<(n + 3) / 4, 256, 0, s>>>(b->Eatom, b->off, s0, s1, m.nout, b->Estruct);
+  k_energy_reduce<<<n, 256, 0, s>>>(b->Eatom, b->off, s0, m.nout, b->Estruct);
   return hipGetLastError();
 }
 
