@@ -221,8 +221,27 @@ def parse_hsd(path):
     info["delta"] = float(m.group(1)) if m else 0.0
     m = re.search(r"Shuffle\s*=\s*(\w+)", txt)
     info["shuffle"] = m.group(1).lower() if m else "no"
-    m = re.search(r"Regularization|Regularisation", txt)
+    m = re.search(r"Regularization\s*=\s*(\w+)", txt)
     info["regularization"] = 1 if m else 0
+    info["regu_type"] = m.group(1).lower() if m else ""
+    m = re.search(r"Strength\s*=\s*([0-9.eE+-]+)", txt)
+    info["regu_strength"] = float(m.group(1)) if m else 0.0
+    m = re.search(r"Alpha\s*=\s*([0-9.eE+-]+)", txt)
+    info["regu_alpha"] = float(m.group(1)) if m else 0.0
+    m = re.search(r"RandomSeed\s*=\s*([0-9]+)", txt)
+    info["seed"] = int(m.group(1)) if m else 0
+    m = re.search(r"Threshold\s*=\s*([0-9.eE+-]+)", txt)
+    info["threshold"] = float(m.group(1)) if m else 0.0
+    m = re.search(r"MinDisplacement\s*=\s*([0-9.eE+-]+)", txt)
+    info["mindisp"] = float(m.group(1)) if m else 1e-6
+    m = re.search(r"NIterations\s*=\s*([0-9]+)", txt)
+    info["niterations"] = int(m.group(1)) if m else 0
+    m = re.search(r"Memory\s*=\s*([0-9]+)", txt)
+    info["memory"] = int(m.group(1)) if m else 1000
+    m = re.search(r"Linemin\s*=\s*(\w+)", txt)
+    info["linemin"] = 0 if (m and m.group(1).lower() == "no") else 1
+    m = re.search(r"ReadNetStats\s*=\s*(\w+)", txt)
+    info["readnetstats"] = 1 if (m and m.group(1).lower() == "yes") else 0
     return info
 
 
@@ -233,24 +252,36 @@ def main():
         d = export_dataset(os.path.join(REF, "datasets", fn))
         np.savez_compressed(os.path.join(OUT, "datasets", fn.replace(".hdf5", ".npz")), **d)
     n = 0
-    roots = ["analysis", "prediction", "validation", "restart"]
+    roots = ["analysis", "prediction", "validation", "restart", "input", "training"]
     for root in roots:
         for dirpath, _, files in sorted(os.walk(os.path.join(REF, root))):
-            if "fortnet_in.hsd" not in files or "fortnet.hdf5" not in files:
+            if "fortnet_in.hsd" not in files:
                 continue
             rel = os.path.relpath(dirpath, REF)
             info = parse_hsd(os.path.join(dirpath, "fortnet_in.hsd"))
+            # training from scratch (RANLUX-initialised weights) has no input netstat
+            scratch = info["mode"] == "train" and not info["readnetstats"]
+            if "fortnet.hdf5" not in files and not scratch:
+                continue
+            if scratch and "_fortnet.hdf5" not in files:
+                continue
             case = {"name": rel, "dataset": info["dataset"], "mode": info["mode"],
                     "optimizer": info["optimizer"], "learningrate": info["learningrate"],
                     "maxdisp": info["maxdisp"],
                     "loss": info["loss"], "forces_kind": info["forces"], "delta": info["delta"],
-                    "shuffle": info["shuffle"], "regularization": info["regularization"]}
-            for k, v in export_netstat(os.path.join(dirpath, "fortnet.hdf5")).items():
-                case["in_" + k] = v
+                    "shuffle": info["shuffle"], "regularization": info["regularization"],
+                    "regu_type": info["regu_type"], "regu_strength": info["regu_strength"],
+                    "regu_alpha": info["regu_alpha"], "seed": info["seed"],
+                    "threshold": info["threshold"], "mindisp": info["mindisp"],
+                    "niterations": info["niterations"], "memory": info["memory"],
+                    "linemin": info["linemin"], "scratch": int(scratch)}
+            if "fortnet.hdf5" in files:
+                for k, v in export_netstat(os.path.join(dirpath, "fortnet.hdf5")).items():
+                    case["in_" + k] = v
             if "_fnetout.hdf5" in files:
                 for k, v in export_fnetout(os.path.join(dirpath, "_fnetout.hdf5")).items():
                     case["out_" + k] = v
-            if root == "restart" and "_fortnet.hdf5" in files:
+            if info["mode"] == "train" and "_fortnet.hdf5" in files:
                 for k, v in export_netstat(os.path.join(dirpath, "_fortnet.hdf5")).items():
                     case["after_" + k] = v
             name = rel.replace("/", "__")
