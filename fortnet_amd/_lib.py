@@ -16,6 +16,7 @@ FNET_FP32 = 1
 FNET_FP32_NET64 = 2
 FNET_BATCHED = 0
 FNET_PER_STRUCT = 1
+LOSSES = {"mse": 0, "mae": 1, "mape": 2, "rms": 3}  # fnet_loss
 
 EXPORTS = [
     "fnet_ctx_create", "fnet_ctx_destroy", "fnet_last_error", "fnet_params_per_species",
@@ -23,7 +24,8 @@ EXPORTS = [
     "fnet_batch_update_coords", "fnet_batch_set_mode", "fnet_batch_n_atoms",
     "fnet_neighbour_list", "fnet_get_neighbours", "fnet_acsf_forward", "fnet_energy",
     "fnet_forces", "fnet_run", "fnet_train_gradients", "fnet_optimizer_step_sd", "fnet_sync",
-    "fnet_stage_times", "fnet_batch_reload", "fnet_query",
+    "fnet_stage_times", "fnet_batch_reload", "fnet_query", "fnet_set_loss",
+    "fnet_set_regularization",
 ]
 
 
@@ -85,6 +87,8 @@ def load():
         "fnet_optimizer_step_sd": (i32, [vp, dbl, dbl, dbl, vp, ctypes.POINTER(i32)]),
         "fnet_sync": (i32, [vp]),
         "fnet_stage_times": (i32, [vp, vp, i32]),
+        "fnet_set_loss": (i32, [vp, i32]),
+        "fnet_set_regularization": (i32, [vp, dbl, dbl]),
         "fnet_batch_reload": (i32, [vp, ctypes.POINTER(BatchDesc)]),
         "fnet_query": (i32, [vp]),
     }

@@ -244,6 +244,24 @@ class Context:
     def batch(self, geos, precision=FNET_FP32, launch=FNET_BATCHED):
         return Batch(self, geos, precision, launch)
 
+    def set_loss(self, loss="mse"):
+        """Loss of the training gradient: 'mse', 'mae', 'mape' or 'rms' (``fnet_set_loss``)."""
+        check(self.h, self.lib.fnet_set_loss(self.h, _lib.LOSSES[loss]), "fnet_set_loss")
+
+    def set_regularization(self, kind=None, strength=0.0, alpha=0.0):
+        """Elastic-net regularisation of the weight gradients (``fnet_set_regularization``):
+        kind 'lasso' (alpha 1), 'ridge' (alpha 0), 'elasticnet' (alpha given) or None."""
+        if kind is None or strength == 0.0:
+            strength, alpha = 0.0, 0.0
+        elif kind == "lasso":
+            alpha = 1.0
+        elif kind == "ridge":
+            alpha = 0.0
+        elif kind != "elasticnet":
+            raise ValueError("invalid regularisation type %r" % kind)
+        check(self.h, self.lib.fnet_set_regularization(self.h, float(strength), float(alpha)),
+              "fnet_set_regularization")
+
     def sd_step(self, lr, max_disp=1e4, tol=0.0):
         """``fnet_optimizer_step_sd`` on the gradient of the last train call."""
         out = np.zeros_like(self.model.params)

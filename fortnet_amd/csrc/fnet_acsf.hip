@@ -770,6 +770,7 @@ template <typename T, bool ADJ>
 hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, hipStream_t s) {
   if (m.maxmem <= 4) return launch_mq<T, ADJ, 4>(b, m, a0, a1, s);
   if (m.maxmem <= 8) return launch_mq<T, ADJ, 8>(b, m, a0, a1, s);
+  if (m.maxmem <= 12) return launch_mq<T, ADJ, 12>(b, m, a0, a1, s);
   return launch_mq<T, ADJ, kMaxMembers>(b, m, a0, a1, s);
 }
 

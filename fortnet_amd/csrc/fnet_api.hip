@@ -1029,7 +1029,7 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   }
   if (!(wsum > 0.0)) return fail(ctx, FNET_ERR_ARG, "datapoint weights sum to zero");
   HIPCHECK(ctx, launch_lossgrad(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
-                                dp_weights ? dpw : nullptr, 1.0 / wsum, s));
+                                dp_weights ? dpw : nullptr, 1.0 / wsum, ctx->loss, s));
   const long long np = m.shape.nparams * m.nspecies;
   if (np > ctx->grad_cap) {
     dfree(ctx->d_grad);
@@ -1039,9 +1039,11 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   }
   HIPCHECK(ctx, hipMemsetAsync(ctx->d_grad, 0, sizeof(double) * np, s));
   HIPCHECK(ctx, launch_mlp_grad(b, m, b->precision != FNET_FP32, ctx->d_grad, s));
+  if (ctx->reg_strength > 0.0)
+    HIPCHECK(ctx, launch_regularize(m, ctx->d_grad, ctx->reg_strength / wsum, ctx->reg_alpha, s));
   double* dloss = dpw + b->S;
-  HIPCHECK(ctx, launch_msloss(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
-                              dp_weights ? dpw : nullptr, dloss, s));
+  HIPCHECK(ctx, launch_loss(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
+                            dp_weights ? dpw : nullptr, ctx->loss, dloss, s));
   if (grad)
     HIPCHECK(ctx, hipMemcpyAsync(grad, ctx->d_grad, sizeof(double) * np, hipMemcpyDeviceToHost, s));
   if (pred)
@@ -1052,6 +1054,23 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   HIPCHECK(ctx, hipStreamSynchronize(s));
   ctx->have_grad = true;
   ctx->last_train_batch = b;
+  return FNET_OK;
+}
+
+int fnet_set_loss(fnet_ctx* ctx, int loss) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (loss < FNET_LOSS_MSE || loss > FNET_LOSS_RMS) return fail(ctx, FNET_ERR_ARG, "invalid loss");
+  ctx->loss = loss;
+  return FNET_OK;
+}
+
+int fnet_set_regularization(fnet_ctx* ctx, double strength, double alpha) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!(strength >= 0.0)) return fail(ctx, FNET_ERR_ARG, "regularisation strength must be >= 0");
+  if (!(alpha >= 0.0 && alpha <= 1.0))
+    return fail(ctx, FNET_ERR_ARG, "elastic net alpha out of bounds [0, 1]");
+  ctx->reg_strength = strength;
+  ctx->reg_alpha = strength > 0.0 ? alpha : 0.0;
   return FNET_OK;
 }
 

@@ -111,6 +111,9 @@ struct fnet_ctx {
   long long grad_cap = 0;
   bool have_grad = false;
   fnet_batch* last_train_batch = nullptr;
+  int loss = 0;               // fnet_loss of the training gradient and loss value
+  double reg_strength = 0.0;  // elastic-net regularisation of the weight gradients
+  double reg_alpha = 0.0;
 };
 
 struct fnet_batch {
@@ -219,11 +222,14 @@ hipError_t launch_mlp_fwd(fnet_batch* b, const DevModel& m, int first_tile, int 
 hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1, hipStream_t s);
 hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                            const double* at, const double* aw, const double* dpw, double inv_wsum,
-                           hipStream_t s);
+                           int loss, hipStream_t s);
 hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad, hipStream_t s);
-hipError_t launch_msloss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
-                         const double* at, const double* aw, const double* dpw, double* out,
-                         hipStream_t s);
+hipError_t launch_loss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
+                       const double* at, const double* aw, const double* dpw, int loss,
+                       double* out, hipStream_t s);
+// elastic-net term (scale = strength / sum of datapoint weights) added to the device gradient
+hipError_t launch_regularize(const DevModel& m, double* grad, double scale, double alpha,
+                             hipStream_t s);
 hipError_t launch_sd_step(double* params, const double* grad, long long n, double lr,
                           double max_disp, double tol, double* scratch, int nspecies,
                           hipStream_t s);

@@ -290,13 +290,26 @@ __global__ void __launch_bounds__(256) k_energy_reduce(const double* __restrict_
   }
 }
 
+// Loss-gradient functions of loss.F90:217-281, elementwise on (prediction, target):
+// mse 2 (p - t); mae and rms (p - t) / |p - t| (the reference writes rms as (p - t) / sqrt((p - t)^2));
+// mape 100 (p - t) / (t^2 |p / t - 1|).
+__device__ __forceinline__ double loss_grad(int loss, double p, double t) {
+  const double d = p - t;
+  switch (loss) {
+    case 1: return d / fabs(d);
+    case 2: return 100.0 * d / (t * t * fabs(p / t - 1.0));
+    case 3: return d / sqrt(d * d);
+    default: return 2.0 * d;
+  }
+}
+
 // Loss gradient of every atomic output with the sysTrain scaling (bpnn.F90:677-700) and the
 // datapoint weight / normalisation of updateGradients + update (bpnn.F90:445-448,766).
 __global__ void k_lossgrad(const double* __restrict__ Eatom, const double* __restrict__ Estruct,
                            const int* __restrict__ atom_struct, const long long* __restrict__ off,
                            long long N, int nout, int nglob, const double* __restrict__ gt,
                            const double* __restrict__ at, const double* __restrict__ aw,
-                           const double* __restrict__ dpw, double inv_wsum,
+                           const double* __restrict__ dpw, double inv_wsum, int loss,
                            double* __restrict__ lg) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -306,23 +319,36 @@ __global__ void k_lossgrad(const double* __restrict__ Eatom, const double* __res
   const int natomic = nout - nglob;
   for (int t = 0; t < nout; ++t) {
     double v;
-    if (t < nglob) v = 2.0 * (Estruct[(long long)s * nout + t] - gt[(long long)s * nglob + t]);
-    else v = 2.0 * (Eatom[i * nout + t] - at[i * natomic + (t - nglob)]);
+    if (t < nglob) v = loss_grad(loss, Estruct[(long long)s * nout + t], gt[(long long)s * nglob + t]);
+    else v = loss_grad(loss, Eatom[i * nout + t], at[i * natomic + (t - nglob)]);
     lg[i * nout + t] = v * sc;
   }
 }
 
-// Mean-squared loss of the batch (msLoss, loss.F90:512-579) in FP64, one block: global targets
-// weighted by w_s * sum(atomic weights of s), atomic targets by w_s * atomic weight; the result
-// is divided by the summed weights.
-__global__ void __launch_bounds__(256) k_msloss(const double* __restrict__ Eatom,
-                                                const double* __restrict__ Estruct,
-                                                const long long* __restrict__ off, int S, int nout,
-                                                int nglob, const double* __restrict__ gt,
-                                                const double* __restrict__ at,
-                                                const double* __restrict__ aw,
-                                                const double* __restrict__ dpw,
-                                                double* __restrict__ out) {
+// Per-item loss of loss.F90:285-366 over the targets of one item (a datapoint's global targets
+// or one atom's atomic targets): mean (t - p)^2, mean |t - p|, 100 mean |(t - p) / t|, or the
+// root of the mean square.
+__device__ __forceinline__ double item_loss(int loss, double sum_sq, double sum_abs,
+                                            double sum_rel, int n) {
+  switch (loss) {
+    case 1: return sum_abs / n;
+    case 2: return 100.0 * sum_rel / n;
+    case 3: return sqrt(sum_sq / n);
+    default: return sum_sq / n;
+  }
+}
+
+// Loss of the batch (maLoss / mapLoss / msLoss / rmsLoss, loss.F90:370-700) in FP64, one block:
+// global targets weighted by w_s * sum(atomic weights of s), atomic targets by w_s * atomic
+// weight; the result is divided by the summed weights.
+__global__ void __launch_bounds__(256) k_loss(const double* __restrict__ Eatom,
+                                              const double* __restrict__ Estruct,
+                                              const long long* __restrict__ off, int S, int nout,
+                                              int nglob, const double* __restrict__ gt,
+                                              const double* __restrict__ at,
+                                              const double* __restrict__ aw,
+                                              const double* __restrict__ dpw, int loss,
+                                              double* __restrict__ out) {
   __shared__ double rl[256], rn[256];
   double l = 0.0, nv = 0.0;
   const int natomic = nout - nglob;
@@ -333,22 +359,28 @@ __global__ void __launch_bounds__(256) k_msloss(const double* __restrict__ Eatom
       const double awi = aw ? aw[i] : 1.0;
       saw += awi;
       if (natomic > 0) {
-        double t = 0.0;
+        double q = 0.0, a = 0.0, r = 0.0;
         for (int o = 0; o < natomic; ++o) {
-          const double d = at[i * natomic + o] - Eatom[i * nout + nglob + o];
-          t += d * d;
+          const double tv = at[i * natomic + o];
+          const double d = tv - Eatom[i * nout + nglob + o];
+          q += d * d;
+          a += fabs(d);
+          r += fabs(d / tv);
         }
-        l += w * awi * (t / natomic);
+        l += w * awi * item_loss(loss, q, a, r, natomic);
         nv += w * awi;
       }
     }
     if (nglob > 0) {
-      double t = 0.0;
+      double q = 0.0, a = 0.0, r = 0.0;
       for (int o = 0; o < nglob; ++o) {
-        const double d = gt[(long long)s * nglob + o] - Estruct[(long long)s * nout + o];
-        t += d * d;
+        const double tv = gt[(long long)s * nglob + o];
+        const double d = tv - Estruct[(long long)s * nout + o];
+        q += d * d;
+        a += fabs(d);
+        r += fabs(d / tv);
       }
-      l += w * saw * (t / nglob);
+      l += w * saw * item_loss(loss, q, a, r, nglob);
       nv += w * saw;
     }
   }
@@ -363,6 +395,21 @@ __global__ void __launch_bounds__(256) k_msloss(const double* __restrict__ Eatom
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = rl[0] / rn[0];
+}
+
+// Elastic-net term of the weight gradients (TWeightDerivs_elasticNetRegularization,
+// nestedtypes.F90:336-370, applied in TBpnn_update before the division by the summed datapoint
+// weights, bpnn.F90:750-766): g_w += scale ((1 - alpha) w + alpha sign(w)) on the weights part
+// of every species' vector, sign(0) = 0; scale = strength / nWeights / sum(w_s).
+__global__ void k_regularize(const double* __restrict__ params, double* __restrict__ grad,
+                             long long nparams, long long nweights, int nspecies, double scale,
+                             double alpha) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nweights * nspecies) return;
+  const long long sp = k / nweights, q = k - sp * nweights;
+  const double w = params[sp * nparams + q];
+  const double sg = w > 0.0 ? 1.0 : (w < 0.0 ? -1.0 : 0.0);
+  grad[sp * nparams + q] += scale * ((1.0 - alpha) * w + alpha * sg);
 }
 
 // Steepest descent of one species' parameter vector (steepdesc.F90:160-204), FP64.
@@ -471,19 +518,31 @@ hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1
 
 hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                            const double* at, const double* aw, const double* dpw, double inv_wsum,
-                           hipStream_t s) {
+                           int loss, hipStream_t s) {
   if (b->N == 0) return hipSuccess;
   k_lossgrad<<<(unsigned)((b->N + 255) / 256), 256, 0, s>>>(b->Eatom, b->Estruct, b->atom_struct,
                                                             b->off, b->N, m.nout, n_global, gt, at,
-                                                            aw, dpw, inv_wsum, b->lossgrad);
+                                                            aw, dpw, inv_wsum, loss, b->lossgrad);
   return hipGetLastError();
 }
 
-hipError_t launch_msloss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
-                         const double* at, const double* aw, const double* dpw, double* out,
-                         hipStream_t s) {
-  k_msloss<<<1, 256, 0, s>>>(b->Eatom, b->Estruct, b->off, b->S, m.nout, n_global, gt, at, aw, dpw,
-                             out);
+hipError_t launch_loss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
+                       const double* at, const double* aw, const double* dpw, int loss,
+                       double* out, hipStream_t s) {
+  k_loss<<<1, 256, 0, s>>>(b->Eatom, b->Estruct, b->off, b->S, m.nout, n_global, gt, at, aw, dpw,
+                           loss, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_regularize(const DevModel& m, double* grad, double scale, double alpha,
+                             hipStream_t s) {
+  long long nw = 0;
+  for (int l = 0; l + 1 < m.shape.nl; ++l) nw += (long long)m.shape.dims[l] * m.shape.dims[l + 1];
+  nw += m.shape.dims[m.shape.nl - 1];  // the dummy output layer ww(d_n, 1)
+  const long long n = nw * m.nspecies;
+  if (n <= 0) return hipSuccess;
+  k_regularize<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m.params, grad, m.shape.nparams, nw,
+                                                           m.nspecies, scale / (double)nw, alpha);
   return hipGetLastError();
 }
 

@@ -144,16 +144,33 @@ int fnet_forces(fnet_ctx* ctx, fnet_batch* batch, double* F);
 int fnet_run(fnet_ctx* ctx, fnet_batch* batch);
 
 /* Training gradient of the whole batch: sysTrain + updateGradients + the normalisation of
- * update (bpnn.F90:394-481,610-704,751-767) with the mse loss gradient (loss.F90:251-264).
+ * update (bpnn.F90:394-481,610-704,751-767) with the loss gradient set by fnet_set_loss
+ * (loss.F90:217-281) and the regularisation set by fnet_set_regularization.
  * g_targets [n_struct][n_global], a_targets [n_atoms][n_out-n_global] (may be NULL when
  * n_out == n_global), atomic_weights [n_atoms] (NULL = 1), dp_weights [n_struct] (NULL = 1).
  * grad [n_species][n_params] receives the FP64 gradient divided by sum(dp_weights);
- * pred [n_atoms][n_out] (may be NULL) the network outputs; loss (may be NULL) the mse loss of
- * msLoss (loss.F90:512-579), FP64. */
+ * pred [n_atoms][n_out] (may be NULL) the network outputs; loss (may be NULL) the loss value
+ * of the data (loss.F90:370-700, without the regularisation term), FP64. */
 int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* batch, int n_global, const double* g_targets,
                          const double* a_targets, const double* atomic_weights,
                          const double* dp_weights, double* grad, double* pred,
                          double* loss);
+
+/* Loss of the training gradient and of the reported loss value (TTraining_setLossFunc,
+ * initprogram.F90:1195-1227; loss.F90:217-700).  Default FNET_LOSS_MSE. */
+typedef enum fnet_loss {
+  FNET_LOSS_MSE = 0,
+  FNET_LOSS_MAE = 1,
+  FNET_LOSS_MAPE = 2,
+  FNET_LOSS_RMS = 3
+} fnet_loss;
+int fnet_set_loss(fnet_ctx* ctx, int loss);
+
+/* Elastic-net regularisation of the weight gradients applied by fnet_train_gradients
+ * (the Regularization block, initprogram.F90:1041-1080: Lasso = alpha 1, Ridge = alpha 0;
+ * TWeightDerivs_elasticNetRegularization, nestedtypes.F90:336-370, as TBpnn_update applies it,
+ * bpnn.F90:750-766).  strength 0 switches it off (default). */
+int fnet_set_regularization(fnet_ctx* ctx, double strength, double alpha);
 
 /* Steepest-descent update in FP64 on the device (steepdesc.F90:160-204):
  * x -= lr*g, scaled to max_disp if max|lr*g| > max_disp; converged if max|g| < tol.

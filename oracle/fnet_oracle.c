@@ -12,9 +12,8 @@
  * fixtures (converted to .npz under tests/golden), by closed forms and by
  * finite differences; see the oracle tests.
  *
- * Parity unpinned: the activations relu, lrelu, softplus, bent, atan, gaussian, heaviside
- * and linear (act_f/act_d) follow transfer.F90:54-342 but no converted fixture exercises
- * them; tanh and sigmoid are pinned by the prediction and restart fixtures.
+ * The ten activations (act_f/act_d, transfer.F90:54-342) are pinned by the reference's
+ * scratch-training fixtures (one steepest-descent step with each transfer function).
  *
  * Layouts (row-major C == Fortran column-major with the indices reversed):
  *   coords[n][3]           Fortran coords(3, nAtom)

@@ -13,9 +13,9 @@ through ctypes and adds the host-side glue of the reference's driver:
 * predictions / global sums / analytic forces, ``forces.F90:317-425``;
 * central finite-difference forces, ``forces.F90:182-313``.
 
-Parity unpinned: network activations other than tanh and sigmoid (no converted fixture uses
-them); everything else is pinned by the reference fixtures, closed forms, invariants or finite
-differences (tests/test_oracle.py).
+Every function is pinned by the reference fixtures, closed forms, invariants or finite
+differences (tests/test_oracle.py); the ten activations by the scratch-training fixtures
+(tests/test_oracle_training.py).
 """
 import ctypes
 import os
