@@ -70,6 +70,22 @@ def test_fnetout_parity(ctx, name, precision):
     b.close()
 
 
+FD_CASES = [n for n in PRED_CASES if "numerical" in n]
+
+
+@pytest.mark.parametrize("name", FD_CASES)
+def test_finite_difference_forces(ctx, name):
+    """The numerical-derivative analysis itself (forces.F90:85-117,182-313): 6 N displaced
+    copies per structure, energies on the device in FP64, central differences; held to the
+    reference's tolerance against its stored finite-difference forces."""
+    from fortnet_amd.forces import finite_difference_forces
+    c, ds = load_case(name)
+    print("case", name, flush=True)
+    ctx.upload_model(fa.Netstat.from_arrays(c, "in_"))
+    F = np.concatenate(finite_difference_forces(ctx, dataset.geometries(ds), float(c["delta"])))
+    _check(F, c["out_forces"], fa.FNET_FP64, "finite-difference forces")
+
+
 @pytest.mark.parametrize("precision", [fa.FNET_FP64, fa.FNET_FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("name", SD_CASES)
 def test_sd_restart_parity(ctx, name, precision):
