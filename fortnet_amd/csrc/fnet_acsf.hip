@@ -14,7 +14,11 @@
 //    once with weight 2 when both lists are the same (the ordered double sum with its j == k
 //    terms, acsf.F90:1420-1432), or the full list1 x list2 product otherwise.  With this order
 //    the 64 lanes of one step touch 64 different entries, which keeps the adjoint's LDS
-//    accumulation free of same-address collisions, and the walk needs no division per step;
+//    accumulation free of same-address collisions, and the walk needs no division per step.
+//    When every group's exponents are powers of two (up to 16) with at most two lambda values,
+//    the kernel is built for that form: the powers come from one squaring chain per lambda, the
+//    same straight-line code for every member (a per-member branch on the exponent cost 25 % of
+//    the adjoint's time on the water boxes); other models use the per-member loop;
 //  * the adjoint pass computes F = -dE/dR directly from c = dE/dG (the network's input gradient
 //    times the z-score factor): the derivative of every function with respect to each entry's
 //    position (gFuncGrad, acsf.F90:1496-1670) is weighted by c and summed per entry, then scattered
@@ -404,7 +408,207 @@ __device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int l
   else pairs_adj<T, MQ, SAME, G4, false>(sl, lane, A0, LA, B0, LB, mb, rc, eta, pi, pirc);
 }
 
-template <typename T, bool ADJ, int MQ>
+// ---- exponent-ladder groups ----
+// Every member has xi = 2^m (m <= MX) and one of two lambda values; slot (i, m) stands for
+// (lambda_i, 2^m).  Per pair and lambda, P = 1 + lambda cos(theta) is squared MX times, which gives
+// P^(2^m) for every slot and, as the running product, P^(2^m - 1) for the derivative: the same
+// straight-line code for every slot, with no per-member branch on the exponent.
+
+// Forward: off-diagonal pairs, FP64 partial sums per slot.
+template <typename T, int MX, bool SAME, bool G4>
+__device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
+                                              int LB, T lam0, T lam1, T rc, T eta, T pi, T pirc,
+                                              double (&acc)[2][MX + 1]) {
+  using V = typename Vec4<T>::type;
+  PairWalk<SAME> it;
+  it.init(lane, A0, LA, B0, LB);
+  int ej, ek;
+  while (it.next(ej, ek)) {
+    const V dj = sl.d[ej], dk = sl.d[ek];
+    const V pj = sl.pa[ej], pk = sl.pa[ek];
+    T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
+    c = fmin(fmax(c, (T)-1), (T)1);
+    T base = (SAME ? (T)2 : (T)1) * pj.y * pk.y;
+    if (G4) {
+      const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
+      const T r2 = rx * rx + ry * ry + rz * rz;
+      const T rjk = fsqrt(r2);
+      const T fcjk = (T)0.5 * pj.z * pk.z * (fcos(cutarg(rjk, pi, rc, pirc)) + (T)1);
+      const T f = fexp(-eta * r2) * fcjk;
+      base = (rjk <= rc) ? base * f : (T)0;
+    }
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      T p = (T)1 + (l ? lam1 : lam0) * c;
+      acc[l][0] += (double)(base * p);
+#pragma unroll
+      for (int m = 1; m <= MX; ++m) {
+        p = p * p;
+        acc[l][m] += (double)(base * p);
+      }
+    }
+  }
+}
+
+// Adjoint: off-diagonal pairs; c0 = dE/dG 2^(1 - xi) of the slot's member (0 for an empty slot),
+// c1 = c0 xi lambda; c1const = the c1 of the xi = 1 slots, whose P^(xi - 1) is 1.
+template <typename T, int MX, bool SAME, bool G4>
+__device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
+                                              int LB, T lam0, T lam1, const T (&c0)[2][MX + 1],
+                                              const T (&c1)[2][MX + 1], T c1const, T rc, T eta,
+                                              T pi, T pirc) {
+  using V = typename Vec4<T>::type;
+  PairWalk<SAME> it;
+  it.init(lane, A0, LA, B0, LB);
+  const T w = SAME ? (T)2 : (T)1;
+  int ej, ek;
+  while (it.next(ej, ek)) {
+    const V dj = sl.d[ej], dk = sl.d[ek];
+    const V pj = sl.pa[ej], pk = sl.pa[ek];
+    const T irj = pj.x, irk = pk.x;
+    const T Ej = pj.y, Epj = pj.w, Ek = pk.y, Epk = pk.w;
+    T H = 1, Hp = 0, irjk = 0;
+    T rx = 0, ry = 0, rz = 0;
+    if (G4) {
+      rx = dj.x - dk.x;
+      ry = dj.y - dk.y;
+      rz = dj.z - dk.z;
+      const T r2 = rx * rx + ry * ry + rz * rz;
+      const T rjk = fsqrt(r2);
+      if (!(rjk <= rc)) continue;
+      const T qq = pj.z * pk.z;
+      const T x = cutarg(rjk, pi, rc, pirc);
+      const T ex = fexp(-eta * r2);
+      const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
+      const T dfc = -(T)0.5 * qq * pirc * fsin(x);
+      H = ex * fc;
+      Hp = ex * (dfc - (T)2 * eta * rjk * fc);
+      irjk = frcp(rjk);
+    }
+    const T ujx = dj.x * irj, ujy = dj.y * irj, ujz = dj.z * irj;
+    const T ukx = dk.x * irk, uky = dk.y * irk, ukz = dk.z * irk;
+    const T c = fmin(fmax(ujx * ukx + ujy * uky + ujz * ukz, (T)-1), (T)1);
+    T S0 = 0, S1 = c1const;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      T p = (T)1 + (l ? lam1 : lam0) * c;
+      T pm = (T)1;
+      S0 += c0[l][0] * p;
+#pragma unroll
+      for (int m = 1; m <= MX; ++m) {
+        pm = pm * p;
+        p = p * p;
+        S0 += c0[l][m] * p;
+        S1 += c1[l][m] * pm;
+      }
+    }
+    const T B = Ej * Ek * H;
+    const T cj = w * S1 * B * irj, ck = w * S1 * B * irk;
+    const T aj = w * S0 * Epj * Ek * H, ak = w * S0 * Ej * Epk * H;
+    T gjx = cj * (ukx - c * ujx) + aj * ujx;
+    T gjy = cj * (uky - c * ujy) + aj * ujy;
+    T gjz = cj * (ukz - c * ujz) + aj * ujz;
+    T gkx = ck * (ujx - c * ukx) + ak * ukx;
+    T gky = ck * (ujy - c * uky) + ak * uky;
+    T gkz = ck * (ujz - c * ukz) + ak * ukz;
+    if (G4) {
+      const T h = w * S0 * Ej * Ek * Hp * irjk;
+      gjx += h * rx;
+      gjy += h * ry;
+      gjz += h * rz;
+      gkx -= h * rx;
+      gky -= h * ry;
+      gkz -= h * rz;
+    }
+    atomicAdd(&sl.gs[3 * ej + 0], (double)gjx);
+    atomicAdd(&sl.gs[3 * ej + 1], (double)gjy);
+    atomicAdd(&sl.gs[3 * ej + 2], (double)gjz);
+    atomicAdd(&sl.gs[3 * ek + 0], (double)gkx);
+    atomicAdd(&sl.gs[3 * ek + 1], (double)gky);
+    atomicAdd(&sl.gs[3 * ek + 2], (double)gkz);
+  }
+}
+
+// One ladder group of one centre: values (forward) or the per-entry adjoint accumulation.
+template <typename T, bool ADJ, int MX>
+__device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, long long i, int A0,
+                          int L1, int B0, int L2, T rc, T eta, T pi, T pirc, const double* cG,
+                          double* G, int nG) {
+  using V = typename Vec4<T>::type;
+  const bool g4 = grp.type == 4;
+  const bool same = grp.same != 0;
+  const T lam0 = (T)grp.lamv[0], lam1 = (T)grp.lamv[1];
+  if (!ADJ) {
+    double acc[2][MX + 1];
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) acc[l][m] = 0.0;
+    if (same) {
+      // j == k terms: cos(theta) = 1, r_jk = 0, fc(0) = 1 (acsf.F90:1420-1432)
+      for (int e = lane; e < L1; e += 64) {
+        const V pj = sl.pa[A0 + e];
+        T base = pj.y * pj.y;
+        if (g4) base *= pj.z * pj.z;
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+#pragma unroll
+          for (int m = 0; m <= MX; ++m) acc[l][m] += (double)(base * (T)grp.lpdiag[5 * l + m]);
+      }
+      if (g4) pairs_fwd_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
+      else pairs_fwd_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
+    } else {
+      if (g4) pairs_fwd_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
+      else pairs_fwd_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
+    }
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) {
+        const int q = grp.smem[5 * l + m];
+        if (q >= 0) {
+          const double s = wave_sum(acc[l][m]);
+          if (lane == 0) G[i * nG + grp.col[q]] = s * grp.scale[q];
+        }
+      }
+  } else {
+    T c0[2][MX + 1], c1[2][MX + 1];
+    T S0d = 0;
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) {
+        const int q = grp.smem[5 * l + m];
+        const T v = q >= 0 ? (T)(cG[grp.col[q]] * grp.scale[q]) : (T)0;
+        c0[l][m] = v;
+        c1[l][m] = v * (T)(1 << m) * (l ? lam1 : lam0);
+        S0d += v * (T)grp.lpdiag[5 * l + m];
+      }
+    const T c1const = c1[0][0] + c1[1][0];
+    if (same) {
+      // j == k terms: only the radial factors depend on r_j; P = 1 + lambda at c = 1
+      // (acsf.F90:1580-1585).  Each lane owns its entries here, so the update is plain.
+      for (int e = lane; e < L1; e += 64) {
+        const int ej = A0 + e;
+        const V dj = sl.d[ej];
+        const V pj = sl.pa[ej];
+        const T H = g4 ? pj.z * pj.z : (T)1;
+        const T gsc = S0d * (T)2 * pj.y * pj.w * H * pj.x;
+        sl.gs[3 * ej + 0] += gsc * dj.x;
+        sl.gs[3 * ej + 1] += gsc * dj.y;
+        sl.gs[3 * ej + 2] += gsc * dj.z;
+      }
+      if (g4) pairs_adj_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
+      else pairs_adj_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
+    } else {
+      if (g4) pairs_adj_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
+      else pairs_adj_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
+    }
+  }
+}
+
+// LAD: every angular group has the exponent-ladder form (AngGroup::lad > 0); MQ is then unused.
+template <typename T, bool ADJ, int MQ, bool LAD>
 __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
@@ -583,11 +787,16 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
       const int nm = grp.nmem;
       T A[MQ];
       if (ADJ) {
+        // a group whose dE/dG are all zero adds nothing to the forces
         bool any = false;
+        if (LAD) {
+          for (int q = 0; q < nm; ++q) any = any || (cG[grp.col[q]] != 0.0);
+        } else {
 #pragma unroll
-        for (int q = 0; q < MQ; ++q) {
-          A[q] = (q < nm) ? (T)(cG[grp.col[q]] * grp.scale[q]) : (T)0;
-          any = any || (A[q] != (T)0);
+          for (int q = 0; q < MQ; ++q) {
+            A[q] = (q < nm) ? (T)(cG[grp.col[q]] * grp.scale[q]) : (T)0;
+            any = any || (A[q] != (T)0);
+          }
         }
         if (!any) continue;
       }
@@ -631,71 +840,82 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
       }
       wave_sync();
 
-      Members<T, MQ> mb;
-      bool ipow = true;
-#pragma unroll
-      for (int q = 0; q < MQ; ++q) {
-        const bool real = q < nm;
-        mb.xint[q] = real ? grp.xint[q] : 0;
-        mb.xi[q] = real ? (T)grp.xi[q] : (T)0;
-        mb.lam[q] = real ? (T)grp.lam[q] : (T)0;
-        ipow = ipow && mb.xint[q] >= 0;
-        mb.pdiag[q] = real ? (T)grp.pdiag[q] : (T)0;
-        mb.A[q] = ADJ ? A[q] : (T)0;
-        mb.AXL[q] = ADJ ? A[q] * mb.xi[q] * mb.lam[q] : (T)0;
-      }
-
-      if (!ADJ) {
-        double acc[MQ];
-#pragma unroll
-        for (int q = 0; q < MQ; ++q) acc[q] = 0.0;
-        if (same) {
-          // j == k terms: cos(theta) = 1, r_jk = 0, fc(0) = 1 (acsf.F90:1420-1432)
-          for (int e = lane; e < L1; e += 64) {
-            const V pj = sl.pa[A0 + e];
-            T base = pj.y * pj.y;
-            if (g4) base *= pj.z * pj.z;
-#pragma unroll
-            for (int q = 0; q < MQ; ++q) acc[q] += (double)(base * mb.pdiag[q]);
-          }
-          if (g4) pairs_fwd_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
-          else pairs_fwd_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
-        } else {
-          if (g4) pairs_fwd_p<T, MQ, false, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
-          else pairs_fwd_p<T, MQ, false, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+      if constexpr (LAD) {
+        switch (grp.lad) {
+          case 1: group_lad<T, ADJ, 1>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          case 2: group_lad<T, ADJ, 2>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          case 3: group_lad<T, ADJ, 3>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          default: group_lad<T, ADJ, 4>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
         }
+        wave_sync();
+        continue;
+      } else {
+        Members<T, MQ> mb;
+        bool ipow = true;
 #pragma unroll
         for (int q = 0; q < MQ; ++q) {
-          if (q < nm) {
-            const double s = wave_sum(acc[q]);
-            if (lane == 0) a.G[i * a.nG + grp.col[q]] = s * grp.scale[q];
-          }
+          const bool real = q < nm;
+          mb.xint[q] = real ? grp.xint[q] : 0;
+          mb.xi[q] = real ? (T)grp.xi[q] : (T)0;
+          mb.lam[q] = real ? (T)grp.lam[q] : (T)0;
+          ipow = ipow && mb.xint[q] >= 0;
+          mb.pdiag[q] = real ? (T)grp.pdiag[q] : (T)0;
+          mb.A[q] = ADJ ? A[q] : (T)0;
+          mb.AXL[q] = ADJ ? A[q] * mb.xi[q] * mb.lam[q] : (T)0;
         }
-      } else {
-        if (same) {
-          // j == k terms: only the radial factors depend on r_j; P = 1 + lambda at c = 1
-          // (acsf.F90:1580-1585).  Each lane owns its entries here, so the update is plain.
-          T S0d = 0;
+
+        if (!ADJ) {
+          double acc[MQ];
 #pragma unroll
-          for (int q = 0; q < MQ; ++q) S0d += mb.A[q] * mb.pdiag[q];
-          for (int e = lane; e < L1; e += 64) {
-            const int ej = A0 + e;
-            const V dj = sl.d[ej];
-            const V pj = sl.pa[ej];
-            const T H = g4 ? pj.z * pj.z : (T)1;
-            const T gsc = S0d * (T)2 * pj.y * pj.w * H * pj.x;
-            sl.gs[3 * ej + 0] += gsc * dj.x;
-            sl.gs[3 * ej + 1] += gsc * dj.y;
-            sl.gs[3 * ej + 2] += gsc * dj.z;
+          for (int q = 0; q < MQ; ++q) acc[q] = 0.0;
+          if (same) {
+            // j == k terms: cos(theta) = 1, r_jk = 0, fc(0) = 1 (acsf.F90:1420-1432)
+            for (int e = lane; e < L1; e += 64) {
+              const V pj = sl.pa[A0 + e];
+              T base = pj.y * pj.y;
+              if (g4) base *= pj.z * pj.z;
+#pragma unroll
+              for (int q = 0; q < MQ; ++q) acc[q] += (double)(base * mb.pdiag[q]);
+            }
+            if (g4) pairs_fwd_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+            else pairs_fwd_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+          } else {
+            if (g4) pairs_fwd_p<T, MQ, false, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
+            else pairs_fwd_p<T, MQ, false, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
           }
-          if (g4) pairs_adj_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
-          else pairs_adj_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+#pragma unroll
+          for (int q = 0; q < MQ; ++q) {
+            if (q < nm) {
+              const double s = wave_sum(acc[q]);
+              if (lane == 0) a.G[i * a.nG + grp.col[q]] = s * grp.scale[q];
+            }
+          }
         } else {
-          if (g4) pairs_adj_p<T, MQ, false, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
-          else pairs_adj_p<T, MQ, false, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+          if (same) {
+            // j == k terms: only the radial factors depend on r_j; P = 1 + lambda at c = 1
+            // (acsf.F90:1580-1585).  Each lane owns its entries here, so the update is plain.
+            T S0d = 0;
+#pragma unroll
+            for (int q = 0; q < MQ; ++q) S0d += mb.A[q] * mb.pdiag[q];
+            for (int e = lane; e < L1; e += 64) {
+              const int ej = A0 + e;
+              const V dj = sl.d[ej];
+              const V pj = sl.pa[ej];
+              const T H = g4 ? pj.z * pj.z : (T)1;
+              const T gsc = S0d * (T)2 * pj.y * pj.w * H * pj.x;
+              sl.gs[3 * ej + 0] += gsc * dj.x;
+              sl.gs[3 * ej + 1] += gsc * dj.y;
+              sl.gs[3 * ej + 2] += gsc * dj.z;
+            }
+            if (g4) pairs_adj_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+            else pairs_adj_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+          } else {
+            if (g4) pairs_adj_p<T, MQ, false, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+            else pairs_adj_p<T, MQ, false, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc);
+          }
         }
+        wave_sync();
       }
-      wave_sync();
     }
 
     // ---- scatter of the adjoint to the forces ----
@@ -728,7 +948,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
   }
 }
 
-template <typename T, bool ADJ, int MQ>
+template <typename T, bool ADJ, int MQ, bool LAD>
 hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a1,
                      hipStream_t s) {
   const long long n = a1 - a0;
@@ -762,16 +982,17 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
-  k_acsf<T, ADJ, MQ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+  k_acsf<T, ADJ, MQ, LAD><<<grid, 64 * wpb, sb * wpb, s>>>(a);
   return hipGetLastError();
 }
 
 template <typename T, bool ADJ>
 hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, hipStream_t s) {
-  if (m.maxmem <= 4) return launch_mq<T, ADJ, 4>(b, m, a0, a1, s);
-  if (m.maxmem <= 8) return launch_mq<T, ADJ, 8>(b, m, a0, a1, s);
-  if (m.maxmem <= 12) return launch_mq<T, ADJ, 12>(b, m, a0, a1, s);
-  return launch_mq<T, ADJ, kMaxMembers>(b, m, a0, a1, s);
+  if (m.all_lad) return launch_mq<T, ADJ, 1, true>(b, m, a0, a1, s);
+  if (m.maxmem <= 4) return launch_mq<T, ADJ, 4, false>(b, m, a0, a1, s);
+  if (m.maxmem <= 8) return launch_mq<T, ADJ, 8, false>(b, m, a0, a1, s);
+  if (m.maxmem <= 12) return launch_mq<T, ADJ, 12, false>(b, m, a0, a1, s);
+  return launch_mq<T, ADJ, kMaxMembers, false>(b, m, a0, a1, s);
 }
 
 }  // namespace

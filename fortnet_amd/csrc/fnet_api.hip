@@ -74,6 +74,45 @@ int zcode_of(const DevModel& m, int z) {
   return -2;
 }
 
+// Fills the exponent-ladder form of an angular group (lad = 0 when the group does not have it):
+// every xi a power of two up to 16, at most two lambda values, each (lambda, xi) once.  The usual
+// ACSF sets (lambda = +-1, xi = 1, 2, 4, 8, 16) have it.
+void ladder_form(AngGroup& a) {
+  a.lad = 0;
+  for (int s = 0; s < kLadSlots; ++s) {
+    a.smem[s] = -1;
+    a.lpdiag[s] = 0.0;
+  }
+  int nl = 0, mx = 1;
+  for (int q = 0; q < a.nmem; ++q) {
+    int m = -1;
+    for (int k = 0; k <= 4; ++k)
+      if (a.xint[q] == (1 << k)) m = k;
+    if (m < 0) return;
+    int i = -1;
+    for (int k = 0; k < nl; ++k)
+      if (a.lamv[k] == a.lam[q]) i = k;
+    if (i < 0) {
+      if (nl == 2) return;
+      i = nl++;
+      a.lamv[i] = a.lam[q];
+    }
+    if (a.smem[5 * i + m] >= 0) return;
+    a.smem[5 * i + m] = q;
+    mx = std::max(mx, m);
+  }
+  if (nl == 0) a.lamv[0] = 0.0;
+  if (nl < 2) a.lamv[1] = a.lamv[0];
+  for (int i = 0; i < 2; ++i) {
+    double p = 1.0 + a.lamv[i];
+    for (int m = 0; m <= 4; ++m) {
+      a.lpdiag[5 * i + m] = p;
+      p = p * p;
+    }
+  }
+  a.lad = mx;
+}
+
 void invalidate(fnet_batch* b) {
   b->have_nl = b->have_G = b->have_E = b->have_F = false;
 }
@@ -561,6 +600,11 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   m.nAG = (int)grp.size();
   m.maxmem = 1;
   for (const AngGroup& a : grp) m.maxmem = std::max(m.maxmem, a.nmem);
+  m.all_lad = 1;
+  for (AngGroup& a : grp) {
+    ladder_form(a);
+    if (a.lad == 0) m.all_lad = 0;
+  }
   m.need_ext = 0;
   for (const RadialFn& f : rad) m.need_ext = std::max(m.need_ext, f.atomid);
   for (const AngGroup& a : grp) m.need_ext = std::max(m.need_ext, a.atomid);

@@ -14,6 +14,7 @@ constexpr int kMaxLayers = 8;   // input + hidden + output layers of one element
 constexpr int kMaxDim = 256;    // widest layer of the generic network kernels
 constexpr int kMaxMembers = 16; // functions sharing one angular neighbour-pair loop
 constexpr int kMaxZ = 16;       // distinct atomic numbers referenced by one model
+constexpr int kLadSlots = 10;   // (lambda, 2^m) slots of an exponent-ladder group: 2 x 5
 constexpr int kMaxOut = 8;      // network outputs (targets)
 constexpr double kPi = 3.14159265358979323846; // lib_dftbp/constants.F90:15
 
@@ -41,6 +42,12 @@ struct AngGroup {
   double lam[kMaxMembers];
   double scale[kMaxMembers]; // 2^(1 - xi)
   double pdiag[kMaxMembers]; // (1 + lam)^xi: the j == k terms, where cos(theta) = 1
+  // Exponent-ladder form (lad > 0): every member has xi = 2^m with m <= lad <= 4 and one of at
+  // most two lambda values; slot 5 i + m stands for (lamv[i], 2^m) and names its member (or -1).
+  int lad;
+  double lamv[2];
+  int smem[kLadSlots];
+  double lpdiag[kLadSlots]; // (1 + lamv[i])^(2^m)
 };
 
 // Network shape of every element network.
@@ -67,6 +74,7 @@ struct StructInfo {
 struct DevModel {
   int nG = 0, nR = 0, nAG = 0, nZ = 0;
   int maxmem = 1;               // largest angular group
+  int all_lad = 0;              // every angular group has the exponent-ladder form
   int need_ext = 0;             // external feature columns the model reads
   int zlist[kMaxZ];
   int nspecies = 0, nout = 0, nin = 0, next_in = 0;

@@ -157,3 +157,54 @@ def test_train_gradients_match_oracle(ctx, name):
         print("%s precision %d: gradient max rel err %.3e" % (name, prec, rel))
         assert rel <= tol
         b.close()
+
+
+@pytest.mark.parametrize("name", ["si64", "water", "ternary"])
+def test_generic_exponent_path(ctx, name):
+    """One more angular function with xi = 3 (same type, cutoff, eta, species and atom id as the
+    last one, so it joins that group) takes the model off the exponent-ladder kernels (every xi a
+    power of two) onto the per-member loop.  The shared features must agree between the two
+    kernels, and the generic model must match the oracle in FP64 and meet the FP32 gates."""
+    print("workload", name, flush=True)
+    geos, ns = _setup(ctx, name)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    G_lad = b.acsf()
+    b.set_mode(fa.FNET_FP32)
+    G_lad32 = b.acsf()
+    b.close()
+    g = ns.g
+    k = int(np.nonzero((g.type == 4) | (g.type == 5))[0][-1])
+    extra = fa.GFunctions([g.type[k]], [g.rcut[k]], eta=[g.eta[k]], lam=[g.lam[k]], xi=[3.0],
+                          atomid=[g.atomid[k]], z1=[g.z1[k]], z2=[g.z2[k]])
+    g2 = fa.GFunctions.concat([g, extra])
+    dims = ns.dims.copy()
+    dims[0] += 1
+    ns2 = fa.Netstat(g2, ns.species_z, dims, ns.activation,
+                     W.random_params(dims, len(ns.species_z), 7))
+    ctx.upload_model(ns2)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    G2 = b.acsf()
+    b.set_mode(fa.FNET_FP32)
+    G2_32 = b.acsf()
+    b.close()
+    scale = max(1.0, np.abs(G_lad).max())
+    assert np.abs(G2[:, :-1] - G_lad).max() <= 1e-12 * scale
+    assert np.abs(G2_32[:, :-1] - G_lad32).max() <= 1e-5 * scale
+    ns2 = W.with_zscore(ns2, G2)
+    ctx.upload_model(ns2)
+    G0, E0, F0 = _oracle_all(geos, ns2)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    G = b.acsf()
+    _, Ea = b.energy()
+    F = b.forces()
+    assert np.abs(G - G0).max() <= 1e-10 * max(1.0, np.abs(G0).max())
+    assert np.abs(Ea - E0).max() <= 1e-10 * max(1.0, np.abs(E0).max())
+    assert np.abs(F - F0).max() <= 1e-9 * max(1.0, np.abs(F0).max())
+    b.set_mode(fa.FNET_FP32)
+    Es32, _ = b.energy()
+    F32 = b.forces()
+    nat = np.array([geo.n for geo in geos])
+    Es64 = np.array([E0[o:o + n].sum() for o, n in zip(np.cumsum(nat) - nat, nat)])
+    assert (np.abs(Es32[:, 0] - Es64) / nat).max() <= E_TOL
+    assert np.sqrt(np.mean((F32 - F0) ** 2)) * EV_BOHR_TO_EV_AA <= F_RMSE_TOL
+    b.close()
