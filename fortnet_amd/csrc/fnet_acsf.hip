@@ -61,6 +61,15 @@ __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float fdiv(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+// r = sqrt(r2) and 1 / r (FP32: one v_rsq_f32)
+__device__ __forceinline__ void dist_inv(float r2, float& r, float& ir) {
+  ir = __builtin_amdgcn_rsqf(r2);
+  r = r2 * ir;
+}
+__device__ __forceinline__ void dist_inv(double r2, double& r, double& ir) {
+  r = sqrt(r2);
+  ir = 1.0 / r;
+}
 __device__ __forceinline__ double fdiv(double x, double y) { return x / y; }
 // argument pi r / rc of the cosine cutoff (FP32: with the wave-uniform pi / rc hoisted)
 __device__ __forceinline__ float cutarg(float r, float pi, float rc, float pirc) {
@@ -100,7 +109,7 @@ struct AcsfArgs {
 
 // Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry, in element
 // order: the displacement and distance d, the pair-loop factors pa = (1/r, exp(-eta r^2) fc,
-// atom-id prefactor, radial derivative), the list-cutoff fc and fc', the FP64 adjoint
+// atom-id prefactor, radial derivative / r), the list-cutoff fc and fc', the FP64 adjoint
 // accumulator (adjoint only), the neighbour index and element code; then the radial partial
 // sums and the start and length of each element's range.
 template <typename T, bool ADJ>
@@ -114,13 +123,20 @@ __host__ __device__ size_t slice_bytes(int M) {
   return (b + 31) & ~size_t(31);
 }
 
+// LDS-typed pointer: its index arithmetic stays 32-bit (a generic pointer's is 64-bit)
+typedef __attribute__((address_space(3))) double lds_f64;
+
+__device__ __forceinline__ void lds_add(lds_f64* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 template <typename T>
 struct Slice {
   using V = typename Vec4<T>::type;
   V* d;
   V* pa;
   T *fc, *dfc;
-  double* gs;  // FP64: ds_add_f64 measured faster than ds_add_f32 here, and exact to more bits
+  lds_f64* gs;  // FP64: ds_add_f64 measured faster than ds_add_f32 here, and exact to more bits
   int *j, *zc;
   double* red;
   int *st, *nz;
@@ -142,7 +158,7 @@ __device__ Slice<T> make_slice(unsigned char* p, int M) {
   p += sizeof(T) * M;
   s.gs = nullptr;
   if (ADJ) {
-    s.gs = (double*)p;
+    s.gs = (lds_f64*)p;
     p += 3 * sizeof(double) * M;
   }
   s.j = (int*)p;
@@ -313,6 +329,68 @@ __device__ __forceinline__ void pairs_fwd(const Slice<T>& sl, int lane, int A0, 
   }
 }
 
+// Per-pair factors of the adjoint: rr = 1 / (r_j r_k), cos(theta), and for G4 the r_jk factor
+// H = exp(-eta r_jk^2) fc(r_jk) with Hq = H' / r_jk (H = 1, Hq = 0 for G5).  False for G4 pairs
+// whose r_jk lies beyond the cutoff (no contribution).
+template <typename T, bool G4>
+__device__ __forceinline__ bool adj_geom(const typename Vec4<T>::type& dj,
+                                         const typename Vec4<T>::type& dk,
+                                         const typename Vec4<T>::type& pj,
+                                         const typename Vec4<T>::type& pk, T rc, T eta, T pi,
+                                         T pirc, T& rr, T& c, T& H, T& Hq) {
+  rr = pj.x * pk.x;
+  c = fmin(fmax((dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * rr, (T)-1), (T)1);
+  H = 1;
+  Hq = 0;
+  if (G4) {
+    const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
+    const T r2 = rx * rx + ry * ry + rz * rz;
+    T rjk, irjk;
+    dist_inv(r2, rjk, irjk);
+    if (!(rjk <= rc)) return false;
+    const T qq = pj.z * pk.z;
+    const T x = cutarg(rjk, pi, rc, pirc);
+    const T ex = fexp(-eta * r2);
+    const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
+    const T dfc = -(T)0.5 * qq * pirc * fsin(x);
+    H = ex * fc;
+    Hq = ex * (dfc - (T)2 * eta * rjk * fc) * irjk;
+  }
+  return true;
+}
+
+// Adds the gradient of one pair to the per-entry accumulators.  S0 = sum over members of
+// dE/dG (1 + lambda c)^xi and S1 = its derivative in c, both times the pair weight.  With the
+// per-entry factors E = exp(-eta r^2) fc(r) (pa.y) and E' / r (pa.w), the gradients are
+//   g_j = Pj d_j + Q d_k,  g_k = Pk d_k + Q d_j,
+//   Pj = S0 H E_k E'_j / r_j - c t / r_j^2 + h,  Q = t / (r_j r_k) - h,
+// with t = S1 E_j E_k H and h = S0 E_j E_k H' / r_jk (the derivative of
+// S0 * E_j E_k H through c, r_j, r_k and r_jk; gFuncGrad, acsf.F90:1496-1670).
+template <typename T, bool G4>
+__device__ __forceinline__ void adj_scatter(const Slice<T>& sl, int ej, int ek,
+                                            const typename Vec4<T>::type& dj,
+                                            const typename Vec4<T>::type& dk,
+                                            const typename Vec4<T>::type& pj,
+                                            const typename Vec4<T>::type& pk, T rr, T c, T H, T Hq,
+                                            T S0, T S1) {
+  const T EE = pj.y * pk.y;
+  const T t = S1 * EE * H;
+  const T U = S0 * H;
+  const T h = G4 ? S0 * EE * Hq : (T)0;
+  const T Q = t * rr - h;
+  const T ct = c * t;
+  const T Pj = U * pk.y * pj.w + h - ct * (pj.x * pj.x);
+  const T Pk = U * pj.y * pk.w + h - ct * (pk.x * pk.x);
+  lds_f64* gj = sl.gs + __umul24(ej, 3);
+  lds_f64* gk = sl.gs + __umul24(ek, 3);
+  lds_add(gj + 0, (double)(Pj * dj.x + Q * dk.x));
+  lds_add(gj + 1, (double)(Pj * dj.y + Q * dk.y));
+  lds_add(gj + 2, (double)(Pj * dj.z + Q * dk.z));
+  lds_add(gk + 0, (double)(Pk * dk.x + Q * dj.x));
+  lds_add(gk + 1, (double)(Pk * dk.y + Q * dj.y));
+  lds_add(gk + 2, (double)(Pk * dk.z + Q * dj.z));
+}
+
 // Adjoint: off-diagonal pairs of one group, gradients accumulated per entry in sl.gs.
 template <typename T, int MQ, bool SAME, bool G4, bool IPOW>
 __device__ __forceinline__ void pairs_adj(const Slice<T>& sl, int lane, int A0, int LA, int B0,
@@ -326,29 +404,8 @@ __device__ __forceinline__ void pairs_adj(const Slice<T>& sl, int lane, int A0, 
   while (it.next(ej, ek)) {
     const V dj = sl.d[ej], dk = sl.d[ek];
     const V pj = sl.pa[ej], pk = sl.pa[ek];
-    const T irj = pj.x, irk = pk.x;
-    const T Ej = pj.y, Epj = pj.w, Ek = pk.y, Epk = pk.w;
-    T H = 1, Hp = 0, irjk = 0;
-    T rx = 0, ry = 0, rz = 0;
-    if (G4) {
-      rx = dj.x - dk.x;
-      ry = dj.y - dk.y;
-      rz = dj.z - dk.z;
-      const T r2 = rx * rx + ry * ry + rz * rz;
-      const T rjk = fsqrt(r2);
-      if (!(rjk <= rc)) continue;
-      const T qq = pj.z * pk.z;
-      const T x = cutarg(rjk, pi, rc, pirc);
-      const T ex = fexp(-eta * r2);
-      const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
-      const T dfc = -(T)0.5 * qq * pirc * fsin(x);
-      H = ex * fc;
-      Hp = ex * (dfc - (T)2 * eta * rjk * fc);
-      irjk = frcp(rjk);
-    }
-    const T ujx = dj.x * irj, ujy = dj.y * irj, ujz = dj.z * irj;
-    const T ukx = dk.x * irk, uky = dk.y * irk, ukz = dk.z * irk;
-    const T c = fmin(fmax(ujx * ukx + ujy * uky + ujz * ukz, (T)-1), (T)1);
+    T rr, c, H, Hq;
+    if (!adj_geom<T, G4>(dj, dk, pj, pk, rc, eta, pi, pirc, rr, c, H, Hq)) continue;
     T S0 = 0, S1 = 0;
 #pragma unroll
     for (int q = 0; q < MQ; ++q) {
@@ -365,30 +422,7 @@ __device__ __forceinline__ void pairs_adj(const Slice<T>& sl, int lane, int A0, 
       S0 += mb.A[q] * pw;
       S1 += mb.AXL[q] * pm1;
     }
-    const T B = Ej * Ek * H;
-    const T cj = w * S1 * B * irj, ck = w * S1 * B * irk;
-    const T aj = w * S0 * Epj * Ek * H, ak = w * S0 * Ej * Epk * H;
-    T gjx = cj * (ukx - c * ujx) + aj * ujx;
-    T gjy = cj * (uky - c * ujy) + aj * ujy;
-    T gjz = cj * (ukz - c * ujz) + aj * ujz;
-    T gkx = ck * (ujx - c * ukx) + ak * ukx;
-    T gky = ck * (ujy - c * uky) + ak * uky;
-    T gkz = ck * (ujz - c * ukz) + ak * ukz;
-    if (G4) {
-      const T h = w * S0 * Ej * Ek * Hp * irjk;
-      gjx += h * rx;
-      gjy += h * ry;
-      gjz += h * rz;
-      gkx -= h * rx;
-      gky -= h * ry;
-      gkz -= h * rz;
-    }
-    atomicAdd(&sl.gs[3 * ej + 0], (double)gjx);
-    atomicAdd(&sl.gs[3 * ej + 1], (double)gjy);
-    atomicAdd(&sl.gs[3 * ej + 2], (double)gjz);
-    atomicAdd(&sl.gs[3 * ek + 0], (double)gkx);
-    atomicAdd(&sl.gs[3 * ek + 1], (double)gky);
-    atomicAdd(&sl.gs[3 * ek + 2], (double)gkz);
+    adj_scatter<T, G4>(sl, ej, ek, dj, dk, pj, pk, rr, c, H, Hq, w * S0, w * S1);
   }
 }
 
@@ -450,8 +484,9 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
   }
 }
 
-// Adjoint: off-diagonal pairs; c0 = dE/dG 2^(1 - xi) of the slot's member (0 for an empty slot),
-// c1 = c0 xi lambda; c1const = the c1 of the xi = 1 slots, whose P^(xi - 1) is 1.
+// Adjoint: off-diagonal pairs; c0 = w dE/dG 2^(1 - xi) of the slot's member (0 for an empty
+// slot, w the pair weight), c1 = c0 xi lambda; c1const = the c1 of the xi = 1 slots, whose
+// P^(xi - 1) is 1.
 template <typename T, int MX, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, const T (&c0)[2][MX + 1],
@@ -460,34 +495,12 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
   using V = typename Vec4<T>::type;
   PairWalk<SAME> it;
   it.init(lane, A0, LA, B0, LB);
-  const T w = SAME ? (T)2 : (T)1;
   int ej, ek;
   while (it.next(ej, ek)) {
     const V dj = sl.d[ej], dk = sl.d[ek];
     const V pj = sl.pa[ej], pk = sl.pa[ek];
-    const T irj = pj.x, irk = pk.x;
-    const T Ej = pj.y, Epj = pj.w, Ek = pk.y, Epk = pk.w;
-    T H = 1, Hp = 0, irjk = 0;
-    T rx = 0, ry = 0, rz = 0;
-    if (G4) {
-      rx = dj.x - dk.x;
-      ry = dj.y - dk.y;
-      rz = dj.z - dk.z;
-      const T r2 = rx * rx + ry * ry + rz * rz;
-      const T rjk = fsqrt(r2);
-      if (!(rjk <= rc)) continue;
-      const T qq = pj.z * pk.z;
-      const T x = cutarg(rjk, pi, rc, pirc);
-      const T ex = fexp(-eta * r2);
-      const T fc = (T)0.5 * qq * (fcos(x) + (T)1);
-      const T dfc = -(T)0.5 * qq * pirc * fsin(x);
-      H = ex * fc;
-      Hp = ex * (dfc - (T)2 * eta * rjk * fc);
-      irjk = frcp(rjk);
-    }
-    const T ujx = dj.x * irj, ujy = dj.y * irj, ujz = dj.z * irj;
-    const T ukx = dk.x * irk, uky = dk.y * irk, ukz = dk.z * irk;
-    const T c = fmin(fmax(ujx * ukx + ujy * uky + ujz * ukz, (T)-1), (T)1);
+    T rr, c, H, Hq;
+    if (!adj_geom<T, G4>(dj, dk, pj, pk, rc, eta, pi, pirc, rr, c, H, Hq)) continue;
     T S0 = 0, S1 = c1const;
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
@@ -502,30 +515,7 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
         S1 += c1[l][m] * pm;
       }
     }
-    const T B = Ej * Ek * H;
-    const T cj = w * S1 * B * irj, ck = w * S1 * B * irk;
-    const T aj = w * S0 * Epj * Ek * H, ak = w * S0 * Ej * Epk * H;
-    T gjx = cj * (ukx - c * ujx) + aj * ujx;
-    T gjy = cj * (uky - c * ujy) + aj * ujy;
-    T gjz = cj * (ukz - c * ujz) + aj * ujz;
-    T gkx = ck * (ujx - c * ukx) + ak * ukx;
-    T gky = ck * (ujy - c * uky) + ak * uky;
-    T gkz = ck * (ujz - c * ukz) + ak * ukz;
-    if (G4) {
-      const T h = w * S0 * Ej * Ek * Hp * irjk;
-      gjx += h * rx;
-      gjy += h * ry;
-      gjz += h * rz;
-      gkx -= h * rx;
-      gky -= h * ry;
-      gkz -= h * rz;
-    }
-    atomicAdd(&sl.gs[3 * ej + 0], (double)gjx);
-    atomicAdd(&sl.gs[3 * ej + 1], (double)gjy);
-    atomicAdd(&sl.gs[3 * ej + 2], (double)gjz);
-    atomicAdd(&sl.gs[3 * ek + 0], (double)gkx);
-    atomicAdd(&sl.gs[3 * ek + 1], (double)gky);
-    atomicAdd(&sl.gs[3 * ek + 2], (double)gkz);
+    adj_scatter<T, G4>(sl, ej, ek, dj, dk, pj, pk, rr, c, H, Hq, S0, S1);
   }
 }
 
@@ -574,14 +564,15 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
   } else {
     T c0[2][MX + 1], c1[2][MX + 1];
     T S0d = 0;
+    const T w = same ? (T)2 : (T)1;  // unordered pairs of one list count twice
 #pragma unroll
     for (int l = 0; l < 2; ++l)
 #pragma unroll
       for (int m = 0; m <= MX; ++m) {
         const int q = grp.smem[5 * l + m];
         const T v = q >= 0 ? (T)(cG[grp.col[q]] * grp.scale[q]) : (T)0;
-        c0[l][m] = v;
-        c1[l][m] = v * (T)(1 << m) * (l ? lam1 : lam0);
+        c0[l][m] = w * v;
+        c1[l][m] = w * v * (T)(1 << m) * (l ? lam1 : lam0);
         S0d += v * (T)grp.lpdiag[5 * l + m];
       }
     const T c1const = c1[0][0] + c1[1][0];
@@ -593,7 +584,7 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
         const V dj = sl.d[ej];
         const V pj = sl.pa[ej];
         const T H = g4 ? pj.z * pj.z : (T)1;
-        const T gsc = S0d * (T)2 * pj.y * pj.w * H * pj.x;
+        const T gsc = S0d * (T)2 * pj.y * pj.w * H;
         sl.gs[3 * ej + 0] += gsc * dj.x;
         sl.gs[3 * ej + 1] += gsc * dj.y;
         sl.gs[3 * ej + 2] += gsc * dj.z;
@@ -810,7 +801,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
       const int B0 = grp.zc2 >= 0 ? sl.st[grp.zc2] : 0;
       const int L2 = grp.zc2 >= 0 ? sl.nz[grp.zc2] : n;
       const T qi = (T)ext_q(a.ext, a.next, grp.atomid, i);
-      // per-entry radial factor exp(-eta r^2) fc(r) of this group (and its r-derivative)
+      // per-entry radial factor E = exp(-eta r^2) fc(r) of this group and E'(r) / r
       for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
         if (e >= n) break;
@@ -835,7 +826,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
         pa.x = frcp(r);
         pa.y = E;
         pa.z = qj;
-        pa.w = Ep;
+        pa.w = Ep * pa.x;
         sl.pa[e] = pa;
       }
       wave_sync();
@@ -902,7 +893,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
               const V dj = sl.d[ej];
               const V pj = sl.pa[ej];
               const T H = g4 ? pj.z * pj.z : (T)1;
-              const T gsc = S0d * (T)2 * pj.y * pj.w * H * pj.x;
+              const T gsc = S0d * (T)2 * pj.y * pj.w * H;
               sl.gs[3 * ej + 0] += gsc * dj.x;
               sl.gs[3 * ej + 1] += gsc * dj.y;
               sl.gs[3 * ej + 2] += gsc * dj.z;
