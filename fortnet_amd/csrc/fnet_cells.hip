@@ -118,84 +118,109 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
   const int b0 = mi.w / (nb1 * nb2), b1 = (mi.w / nb2) % nb1, b2 = mi.w % nb2;
   const int N1 = si.hi[1] - si.lo[1] + 1, N2 = si.hi[2] - si.lo[2] + 1;
   const int origin = ((0 - si.lo[0]) * N1 + (0 - si.lo[1])) * N2 + (0 - si.lo[2]);
+  // The 27 stencil bins, one per lane 0..26: first candidate in cell_atoms, count, and the
+  // periodic wrap (s0, s1, s2) packed as 9 (s0 + 1) + 3 (s1 + 1) + (s2 + 1).  Their candidates are
+  // then taken 64 at a time as one list (a bin holds ~10 atoms, a fraction of a wave); the order
+  // does not matter, the fill pass sorts each row.
+  int bc0 = 0, bcc = 0, bsg = 0;
+  if (lane < 27) {
+    const int d0 = lane / 9 - 1, d1 = (lane / 3) % 3 - 1, d2 = lane % 3 - 1;
+    int B0 = b0 + d0, B1 = b1 + d1, B2 = b2 + d2;
+    const int s0 = B0 < 0 ? -1 : (B0 >= nb0 ? 1 : 0);
+    const int s1 = B1 < 0 ? -1 : (B1 >= nb1 ? 1 : 0);
+    const int s2 = B2 < 0 ? -1 : (B2 >= nb2 ? 1 : 0);
+    B0 -= s0 * nb0;
+    B1 -= s1 * nb1;
+    B2 -= s2 * nb2;
+    const int cell = a.cell_off[s] + (B0 * nb1 + B1) * nb2 + B2;
+    bc0 = a.cell_start[cell];
+    bcc = a.cell_start[cell + 1] - bc0;
+    bsg = (s0 + 1) * 9 + (s1 + 1) * 3 + (s2 + 1);
+  }
+  int incl = bcc;
+#pragma unroll
+  for (int d = 1; d < 32; d <<= 1) {
+    const int t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  const int bex = incl - bcc;  // first flat index of the lane's bin
+  const int total = __shfl(incl, 26, 64);
   int count = 0;
-  for (int d0 = -1; d0 <= 1; ++d0)
-    for (int d1 = -1; d1 <= 1; ++d1)
-      for (int d2 = -1; d2 <= 1; ++d2) {
-        int B0 = b0 + d0, B1 = b1 + d1, B2 = b2 + d2;
-        const int s0 = B0 < 0 ? -1 : (B0 >= nb0 ? 1 : 0);
-        const int s1 = B1 < 0 ? -1 : (B1 >= nb1 ? 1 : 0);
-        const int s2 = B2 < 0 ? -1 : (B2 >= nb2 ? 1 : 0);
-        B0 -= s0 * nb0;
-        B1 -= s1 * nb1;
-        B2 -= s2 * nb2;
-        const int cell = a.cell_off[s] + (B0 * nb1 + B1) * nb2 + B2;
-        const int c0 = a.cell_start[cell], cc = a.cell_start[cell + 1] - c0;
-        for (int p0 = 0; p0 < cc; p0 += 64) {
-          const int p = p0 + lane;
-          bool hit = false;
-          int lj = 0, img = -1;
-          double nd0 = 0, nd1 = 0, nd2 = 0, dd = 0;
-          unsigned long long k = 0;
-          if (p < cc) {
-            const int j = a.cell_atoms[c0 + p];
-            lj = (int)(j - o);
-            const int4 mj = a.atom_m[j];
-            const int n0 = mi.x - mj.x + s0, n1 = mi.y - mj.y + s1, n2 = mi.z - mj.z + s2;
-            const bool inr = n0 >= si.lo[0] && n0 <= si.hi[0] && n1 >= si.lo[1] &&
-                             n1 <= si.hi[1] && n2 >= si.lo[2] && n2 <= si.hi[2];
-            const bool central = (n0 == 0 && n1 == 0 && n2 == 0);
-            if (inr && !(central && lj == li)) {
-              const double pv0 = (double)n0, pv1 = (double)n1, pv2 = (double)n2;
-              double cv[3];
-              for (int r = 0; r < 3; ++r) {
-                double acc = 0.0;
-                acc = acc + L[0 * 3 + r] * pv0;
-                acc = acc + L[1 * 3 + r] * pv1;
-                acc = acc + L[2 * 3 + r] * pv2;
-                cv[r] = acc;
-              }
-              const double nc0 = X[3 * lj] + cv[0], nc1 = X[3 * lj + 1] + cv[1],
-                           nc2 = X[3 * lj + 2] + cv[2];
-              const double e0 = ci0 - nc0, e1 = ci1 - nc1, e2 = ci2 - nc2;
-              double sq = e0 * e0;
-              sq = sq + e1 * e1;
-              sq = sq + e2 * e2;
-              hit = sq <= rc2;
-              nd0 = -e0;
-              nd1 = -e1;
-              nd2 = -e2;
-              dd = sq;
-              int lex = -1;
-              if (!central) {
-                lex = ((n0 - si.lo[0]) * N1 + (n1 - si.lo[1])) * N2 + (n2 - si.lo[2]);
-                if (lex > origin) lex -= 1;
-              }
-              img = lex;
-              int t = lj - li;
-              if (t < 0) t += n;
-              k = (unsigned long long)(lex + 1) * (unsigned long long)n + (unsigned long long)t;
-            }
-          }
-          const unsigned long long mask = __ballot(hit);
-          if (WRITE && hit) {
-            const int q = count + __popcll(mask & lanemask_lt(lane));
-            if (q < K) {
-              key[q] = k;
-              slot[q] = q;
-              pj[q] = (int)(o + lj);
-              pimg[q] = img;
-              V v;
-              v.x = nd0;
-              v.y = nd1;
-              v.z = nd2;
-              v.w = sqrt(dd);
-              pnd[q] = v;
-            }
-          }
-          count += __popcll(mask);
+  for (int p0 = 0; p0 < total; p0 += 64) {
+    const int p = p0 + lane;
+    // bin of candidate p: the last bin whose first flat index is <= p (empty bins share their
+    // successor's index and are passed over)
+    int bb = 0;
+#pragma unroll
+    for (int st = 16; st > 0; st >>= 1) {
+      const int nbb = bb + st;
+      const int e = __shfl(bex, nbb < 27 ? nbb : 26, 64);
+      if (nbb < 27 && e <= p) bb = nbb;
+    }
+    const int c0 = __shfl(bc0, bb, 64), e0b = __shfl(bex, bb, 64), sg = __shfl(bsg, bb, 64);
+    const int s0 = sg / 9 - 1, s1 = (sg / 3) % 3 - 1, s2 = sg % 3 - 1;
+    bool hit = false;
+    int lj = 0, img = -1;
+    double nd0 = 0, nd1 = 0, nd2 = 0, dd = 0;
+    unsigned long long k = 0;
+    if (p < total) {
+      const int j = a.cell_atoms[c0 + (p - e0b)];
+      lj = (int)(j - o);
+      const int4 mj = a.atom_m[j];
+      const int n0 = mi.x - mj.x + s0, n1 = mi.y - mj.y + s1, n2 = mi.z - mj.z + s2;
+      const bool inr = n0 >= si.lo[0] && n0 <= si.hi[0] && n1 >= si.lo[1] &&
+                       n1 <= si.hi[1] && n2 >= si.lo[2] && n2 <= si.hi[2];
+      const bool central = (n0 == 0 && n1 == 0 && n2 == 0);
+      if (inr && !(central && lj == li)) {
+        const double pv0 = (double)n0, pv1 = (double)n1, pv2 = (double)n2;
+        double cv[3];
+        for (int r = 0; r < 3; ++r) {
+          double acc = 0.0;
+          acc = acc + L[0 * 3 + r] * pv0;
+          acc = acc + L[1 * 3 + r] * pv1;
+          acc = acc + L[2 * 3 + r] * pv2;
+          cv[r] = acc;
         }
+        const double nc0 = X[3 * lj] + cv[0], nc1 = X[3 * lj + 1] + cv[1],
+                     nc2 = X[3 * lj + 2] + cv[2];
+        const double e0 = ci0 - nc0, e1 = ci1 - nc1, e2 = ci2 - nc2;
+        double sq = e0 * e0;
+        sq = sq + e1 * e1;
+        sq = sq + e2 * e2;
+        hit = sq <= rc2;
+        nd0 = -e0;
+        nd1 = -e1;
+        nd2 = -e2;
+        dd = sq;
+        int lex = -1;
+        if (!central) {
+          lex = ((n0 - si.lo[0]) * N1 + (n1 - si.lo[1])) * N2 + (n2 - si.lo[2]);
+          if (lex > origin) lex -= 1;
+        }
+        img = lex;
+        int t = lj - li;
+        if (t < 0) t += n;
+        k = (unsigned long long)(lex + 1) * (unsigned long long)n + (unsigned long long)t;
       }
+    }
+    const unsigned long long mask = __ballot(hit);
+    if (WRITE && hit) {
+      const int q = count + __popcll(mask & lanemask_lt(lane));
+      if (q < K) {
+        key[q] = k;
+        slot[q] = q;
+        pj[q] = (int)(o + lj);
+        pimg[q] = img;
+        V v;
+        v.x = nd0;
+        v.y = nd1;
+        v.z = nd2;
+        v.w = sqrt(dd);
+        pnd[q] = v;
+      }
+    }
+    count += __popcll(mask);
+  }
   if (!WRITE) {
     if (lane == 0) a.cnt[i] = count;
     return;

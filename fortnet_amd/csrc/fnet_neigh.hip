@@ -183,6 +183,12 @@ __global__ void __launch_bounds__(256)
   }
   const StructInfo& si = sinfo[s];
   if (si.periodic) {
+    // Lattice images in the reference's lexicographic order (third index fastest, origin
+    // skipped), 64 at a time: the ones that can hold a neighbour are compacted in order into an
+    // LDS table with their translation, then the (image, atom) pairs of the table are tested as
+    // one list, 64 per step, so small cells (n << 64) still fill the wave.
+    __shared__ double4 tab[4][64];  // per wave: translation L . n (x, y, z), image index (w)
+    double4* tw = tab[(threadIdx.x >> 6) & 3];
     const double* L = lat + 9 * s;
     double sf[3];
     for (int a = 0; a < 3; ++a)
@@ -193,35 +199,62 @@ __global__ void __launch_bounds__(256)
       wlo[a] = sf[a] - si.smax[a] - si.w[a] - m;
       whi[a] = sf[a] - si.smin[a] + si.w[a] + m;
     }
-    int img = 0;
-    for (int p1 = si.lo[0]; p1 <= si.hi[0]; ++p1)
-      for (int p2 = si.lo[1]; p2 <= si.hi[1]; ++p2)
-        for (int p3 = si.lo[2]; p3 <= si.hi[2]; ++p3) {
-          if (p1 == 0 && p2 == 0 && p3 == 0) continue;
-          const int me = img++;
-          if (p1 < wlo[0] || p1 > whi[0] || p2 < wlo[1] || p2 > whi[1] || p3 < wlo[2] ||
-              p3 > whi[2])
-            continue;
-          const double pv0 = (double)p1, pv1 = (double)p2, pv2 = (double)p3;
-          double cv[3];
-          for (int r = 0; r < 3; ++r) {
-            double acc = 0.0;
-            acc = acc + L[0 * 3 + r] * pv0;
-            acc = acc + L[1 * 3 + r] * pv1;
-            acc = acc + L[2 * 3 + r] * pv2;
-            cv[r] = acc;
-          }
-          for (int t0 = 0; t0 < n; t0 += 64) {
-            const int t = t0 + lane;
-            int j = li + t;
-            if (j >= n) j -= n;
-            double nd0 = 0, nd1 = 0, nd2 = 0, d2 = 0;
-            bool hit = false;
-            if (t < n)
-              hit = near_test(ci0, ci1, ci2, X, j, cv[0], cv[1], cv[2], rc2, nd0, nd1, nd2, d2);
-            emit(hit, j, me, nd0, nd1, nd2, d2);
-          }
+    const int m1 = si.hi[0] - si.lo[0] + 1, m2 = si.hi[1] - si.lo[1] + 1,
+              m3 = si.hi[2] - si.lo[2] + 1;
+    const int nq = m1 * m2 * m3;
+    const int qorig = ((0 - si.lo[0]) * m2 + (0 - si.lo[1])) * m3 + (0 - si.lo[2]);
+    const int qs = 64 / n, rs = 64 % n;
+    for (int q0 = 0; q0 < nq; q0 += 64) {
+      const int q = q0 + lane;
+      bool keep = false;
+      double4 tv = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (q < nq && q != qorig) {
+        const int p1 = si.lo[0] + q / (m2 * m3), p2 = si.lo[1] + (q / m3) % m2,
+                  p3 = si.lo[2] + q % m3;
+        keep = !(p1 < wlo[0] || p1 > whi[0] || p2 < wlo[1] || p2 > whi[1] || p3 < wlo[2] ||
+                 p3 > whi[2]);
+        const double pv0 = (double)p1, pv1 = (double)p2, pv2 = (double)p3;
+        double cv[3];
+        for (int r = 0; r < 3; ++r) {
+          double acc = 0.0;
+          acc = acc + L[0 * 3 + r] * pv0;
+          acc = acc + L[1 * 3 + r] * pv1;
+          acc = acc + L[2 * 3 + r] * pv2;
+          cv[r] = acc;
         }
+        tv = make_double4(cv[0], cv[1], cv[2], (double)(q > qorig ? q - 1 : q));
+      }
+      const unsigned long long km = __ballot(keep);
+      if (keep) tw[__popcll(km & lanemask_lt(lane))] = tv;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // flat index u = slot n + t over the kept images, t = 0 .. n - 1 from the centre on
+      const int tot = __popcll(km) * n;
+      int slot = lane / n, t = lane % n;
+      for (int u0 = 0; u0 < tot; u0 += 64) {
+        double nd0 = 0, nd1 = 0, nd2 = 0, d2 = 0;
+        bool hit = false;
+        int j = 0, img = 0;
+        if (u0 + lane < tot) {
+          const double4 c = tw[slot];
+          j = li + t;
+          if (j >= n) j -= n;
+          img = (int)c.w;
+          hit = near_test(ci0, ci1, ci2, X, j, c.x, c.y, c.z, rc2, nd0, nd1, nd2, d2);
+        }
+        emit(hit, j, img, nd0, nd1, nd2, d2);
+        slot += qs;
+        t += rs;
+        if (t >= n) {
+          t -= n;
+          slot += 1;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
   if (!WRITE && lane == 0) cnt[i] = count;
 }

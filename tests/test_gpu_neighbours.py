@@ -66,10 +66,10 @@ def test_cluster_exact(ctx):
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_small_cells_with_self_images(ctx, seed):
-    """8..40-atom cells shorter than the cutoff (many images per neighbour, self images)."""
+    """1..70-atom cells shorter than the cutoff (many images per neighbour, self images)."""
     rng = np.random.default_rng(seed)
     geos = []
-    for n in (1, 2, 8, 13, 40):
+    for n in (1, 2, 8, 13, 40, 70):
         geos.append(workloads.random_cell(rng, n, 2.33, 2.0, 14))
     _compare(ctx, geos, 6.0 * fa.AA__BOHR)
 
