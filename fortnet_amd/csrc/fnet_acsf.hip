@@ -109,18 +109,24 @@ struct AcsfArgs {
 
 // Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry, in element
 // order: the displacement and distance d, the pair-loop factors pa = (1/r, exp(-eta r^2) fc,
-// atom-id prefactor, radial derivative / r), the list-cutoff fc and fc', the FP64 adjoint
-// accumulator (adjoint only), the neighbour index and element code; then the radial partial
-// sums and the start and length of each element's range.
+// atom-id prefactor, radial derivative / r), then the radial partial sums and the list-cutoff fc
+// (forward only) or the FP64 adjoint accumulator (adjoint only), the neighbour index and element
+// code, and the start and length of each element's range.  The slice is kept small: the number of
+// resident waves per CU, and with it the adjoint's speed, follows the LDS it takes (a slice 1.5x
+// larger made the adjoint 1.3x slower on the water boxes).
 template <typename T, bool ADJ>
 __host__ __device__ size_t slice_bytes(int M) {
   size_t b = 2 * sizeof(typename Vec4<T>::type) * M;  // d, pa
-  b += 64 * sizeof(double);                           // radial partial sums
-  b += 2 * sizeof(T) * M;                             // fc, dfc
+  if (!ADJ) b += 64 * sizeof(double) + sizeof(T) * M;  // radial partial sums, fc
   if (ADJ) b += 3 * sizeof(double) * M;               // gs
   b += 2 * sizeof(int) * M;                           // j, zc
   b += 2 * sizeof(int) * (kMaxZ + 1);                 // st, nz
   return (b + 31) & ~size_t(31);
+}
+
+// Entries per slice: the largest row, rounded up to a multiple of 8 (keeps every array aligned).
+__host__ __device__ inline int slice_entries(int max_nb) {
+  return max_nb < 8 ? 8 : ((max_nb + 7) / 8) * 8;
 }
 
 // LDS-typed pointer: its index arithmetic stays 32-bit (a generic pointer's is 64-bit)
@@ -135,7 +141,7 @@ struct Slice {
   using V = typename Vec4<T>::type;
   V* d;
   V* pa;
-  T *fc, *dfc;
+  T* fc;  // forward only
   lds_f64* gs;  // FP64: ds_add_f64 measured faster than ds_add_f32 here, and exact to more bits
   int *j, *zc;
   double* red;
@@ -150,12 +156,14 @@ __device__ Slice<T> make_slice(unsigned char* p, int M) {
   p += sizeof(V) * M;
   s.pa = (V*)p;
   p += sizeof(V) * M;
-  s.red = (double*)p;
-  p += 64 * sizeof(double);
-  s.fc = (T*)p;
-  p += sizeof(T) * M;
-  s.dfc = (T*)p;
-  p += sizeof(T) * M;
+  s.red = nullptr;
+  s.fc = nullptr;
+  if (!ADJ) {
+    s.red = (double*)p;
+    p += 64 * sizeof(double);
+    s.fc = (T*)p;
+    p += sizeof(T) * M;
+  }
   s.gs = nullptr;
   if (ADJ) {
     s.gs = (lds_f64*)p;
@@ -673,9 +681,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
       sl.d[pos] = d;
       sl.j[pos] = jj;
       sl.zc[pos] = key < nZ ? key : -1;
-      const T x = cutarg(d.w, pi, rcl, pircl);
-      sl.fc[pos] = (T)0.5 * (fcos(x) + (T)1);
-      sl.dfc[pos] = -(T)0.5 * pircl * fsin(x);
+      if (!ADJ) sl.fc[pos] = (T)0.5 * (fcos(cutarg(d.w, pi, rcl, pircl)) + (T)1);
     }
   }
   wave_sync();
@@ -733,6 +739,9 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
         const V d = sl.d[e];
         const int zc = sl.zc[e];
         const T r = d.w;
+        // list-cutoff fc and fc' of the entry (recomputed rather than kept in LDS)
+        const T xl = cutarg(r, pi, rcl, pircl);
+        const T fcl = (T)0.5 * (fcos(xl) + (T)1), dfcl = -(T)0.5 * pircl * fsin(xl);
         T s = 0;
         for (int f = 0; f < a.nR; ++f) {
           const RadialFn& fn = a.radial[f];
@@ -741,8 +750,8 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
           if (!(fn.zc < 0 || fn.zc == zc) || !(r <= (T)fn.rc)) continue;
           T fc, dfc;
           if (fn.rc == a.rclist && fn.atomid == 0) {
-            fc = sl.fc[e];
-            dfc = sl.dfc[e];
+            fc = fcl;
+            dfc = dfcl;
           } else {
             const T rc = (T)fn.rc;
             const T pirc = pi / rc;
@@ -810,8 +819,14 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
         if (r <= rc) {
           T fc, dfc;
           if (shared) {
-            fc = sl.fc[e];
-            dfc = sl.dfc[e];
+            if (ADJ) {
+              const T x = cutarg(r, pi, rcl, pircl);
+              fc = (T)0.5 * (fcos(x) + (T)1);
+              dfc = -(T)0.5 * pircl * fsin(x);
+            } else {
+              fc = sl.fc[e];
+              dfc = 0;
+            }
           } else {
             qj = (T)ext_q(a.ext, a.next, grp.atomid, sl.j[e]);
             const T x = cutarg(r, pi, rc, pirc);
@@ -820,7 +835,7 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
           }
           const T ex = fexp(-eta * r * r);
           E = ex * fc;
-          Ep = ex * (dfc - (T)2 * eta * r * fc);
+          if (ADJ) Ep = ex * (dfc - (T)2 * eta * r * fc);
         }
         V pa;
         pa.x = frcp(r);
@@ -944,7 +959,7 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
                      hipStream_t s) {
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
-  const int M = ((b->max_nb + 31) / 32) * 32 + 32;
+  const int M = slice_entries(b->max_nb);
   const size_t sb = slice_bytes<T, ADJ>(M);
   // waves per block (water boxes: adjoint 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 /
   // 2.25 ms with 2 / 4)
@@ -990,7 +1005,7 @@ hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, 
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj) {
   (void)nZ;
-  const int M = ((max_nb + 31) / 32) * 32 + 32;
+  const int M = slice_entries(max_nb);
   if (fp64) return adj ? slice_bytes<double, true>(M) : slice_bytes<double, false>(M);
   return adj ? slice_bytes<float, true>(M) : slice_bytes<float, false>(M);
 }
