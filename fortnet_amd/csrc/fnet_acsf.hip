@@ -608,7 +608,7 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
 
 // LAD: every angular group has the exponent-ladder form (AngGroup::lad > 0); MQ is then unused.
 template <typename T, bool ADJ, int MQ, bool LAD>
-__global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
+__device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
   const int lane = threadIdx.x & 63;
@@ -955,6 +955,19 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
 }
 
 template <typename T, bool ADJ, int MQ, bool LAD>
+__global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
+  acsf_wave<T, ADJ, MQ, LAD>(a);
+}
+
+// The FP32 exponent-ladder kernels fit in 96 registers without spilling, which leaves room for 5
+// waves per SIMD instead of 4 (the compiler's choice without the bound).
+template <bool ADJ>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5)))
+    k_acsf_lad32(AcsfArgs a) {
+  acsf_wave<float, ADJ, 1, true>(a);
+}
+
+template <typename T, bool ADJ, int MQ, bool LAD>
 hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a1,
                      hipStream_t s) {
   const long long n = a1 - a0;
@@ -988,7 +1001,10 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
-  k_acsf<T, ADJ, MQ, LAD><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+  if constexpr (LAD && std::is_same<T, float>::value)
+    k_acsf_lad32<ADJ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+  else
+    k_acsf<T, ADJ, MQ, LAD><<<grid, 64 * wpb, sb * wpb, s>>>(a);
   return hipGetLastError();
 }
 
