@@ -212,7 +212,7 @@ hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t
 long long scan_tmp_size(long long n);
 
 // launchers (fnet_cells.hip)
-constexpr int kCellMinAtoms = 1024; // structures at least this large use cell lists
+constexpr int kCellMinAtoms = 64; // structures at least this large use cell lists (if >= 3 bins fit)
 hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s);
 hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,

@@ -88,6 +88,14 @@ def test_cell_list_path_exact(ctx):
     _compare(ctx, [g, g2], 6.0 * fa.AA__BOHR)
 
 
+def test_cell_list_mid_size_cells(ctx):
+    """Cells of a few hundred atoms whose width allows exactly 3 bins per axis: the 27 stencil
+    bins cover the whole cell and every candidate's image comes from the bin wrap."""
+    rng = np.random.default_rng(3)
+    geos = [workloads.random_cell(rng, n, 2.33, 2.0, 14) for n in (300, 420, 512)]
+    _compare(ctx, geos, 6.0 * fa.AA__BOHR)
+
+
 def test_triclinic_and_unwrapped(ctx):
     rng = np.random.default_rng(7)
     L = np.array([[9.0, 0.0, 0.0], [2.5, 8.0, 0.0], [-1.5, 3.0, 10.0]])
