@@ -230,14 +230,45 @@ void ev_record(fnet_ctx* ctx, int k) {
   if (k < 8) (void)hipEventRecord(ctx->ev[k], ctx->stream);
 }
 
+// Room for the one-pass search: nb_cap entries per atom, of either precision.  Returns false
+// (and the caller counts first) when the padded rows would take much more memory than the list.
+bool ensure_padded(fnet_batch* b) {
+  const long long need = (long long)b->N * b->nb_cap;
+  if (need <= 0) return false;
+  if (need > 4 * std::max<long long>(b->npairs, 1) + (1LL << 22)) return false;
+  if (need > b->pad_len) {
+    dfree(b->pnj);
+    dfree(b->pnimg);
+    dfree(b->pnd);
+    b->pnj = b->pnimg = nullptr;
+    b->pnd = nullptr;
+    b->pad_len = 0;
+    if (dalloc(&b->pnj, (size_t)need) != hipSuccess ||
+        dalloc(&b->pnimg, (size_t)need) != hipSuccess ||
+        dalloc((double4**)&b->pnd, (size_t)need) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    b->pad_len = need;
+  }
+  return true;
+}
+
 int do_neighbours(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
   hipStream_t s = ctx->stream;
   const double rc = m.rcmax;
+  const int fp64 = b->precision == FNET_FP64;
   HIPCHECK(ctx, launch_struct_prep(b, rc, s));
   HIPCHECK(ctx, launch_cell_bin(b, s));
-  if (b->launch == FNET_PER_STRUCT) {
+  // One pass into padded rows when the previous list gives a capacity; a row longer than it
+  // sends the batch through the fill pass below, with the counts this pass already made.
+  const bool padded = b->launch == FNET_BATCHED && b->nb_cap > 0 && ensure_padded(b);
+  if (padded) {
+    HIPCHECK(ctx, launch_neigh_pad(b, rc, fp64, s));
+    HIPCHECK(ctx, launch_cells_pad(b, rc, fp64, s));
+  } else if (b->launch == FNET_PER_STRUCT) {
     for (int st = 0; st < b->S; ++st) {
       HIPCHECK(ctx, launch_neigh_count(b, rc, b->h_off[st], b->h_off[st + 1], s));
       HIPCHECK(ctx, launch_cells_count(b, rc, b->h_off[st], b->h_off[st + 1], s));
@@ -273,8 +304,9 @@ int do_neighbours(fnet_batch* b) {
     b->pcap = cap;
   }
   b->npairs = P;
-  const int fp64 = b->precision == FNET_FP64;
-  if (b->launch == FNET_PER_STRUCT) {
+  if (padded && b->max_nb <= b->nb_cap) {
+    HIPCHECK(ctx, launch_compact_rows(b, fp64, s));
+  } else if (b->launch == FNET_PER_STRUCT) {
     for (int st = 0; st < b->S; ++st) {
       HIPCHECK(ctx, launch_neigh_fill(b, rc, b->h_off[st], b->h_off[st + 1], fp64, s));
       HIPCHECK(ctx, launch_cells_fill(b, rc, b->h_off[st], b->h_off[st + 1], fp64, s));
@@ -283,6 +315,8 @@ int do_neighbours(fnet_batch* b) {
     HIPCHECK(ctx, launch_neigh_fill(b, rc, 0, b->N, fp64, s));
     HIPCHECK(ctx, launch_cells_fill(b, rc, 0, b->N, fp64, s));
   }
+  if (b->launch == FNET_BATCHED && (b->nb_cap == 0 || b->max_nb > b->nb_cap))
+    b->nb_cap = ((b->max_nb + b->max_nb / 4 + 8) + 7) / 8 * 8;
   b->have_nl = true;
   b->result_precision = b->precision;
   return FNET_OK;
@@ -858,7 +892,8 @@ void fnet_batch_destroy(fnet_batch* b) {
                   b->rowptr, b->scan_tmp, b->nj, b->nimg, b->nd, b->d_maxnb, b->G, b->cG,
                   b->Eatom, b->Estruct, b->F, b->lossgrad, b->d_targets, b->cellinfo,
                   b->cell_off, b->cell_count, b->cell_start, b->cell_atoms, b->atom_cell,
-                  b->atom_m, b->cell_scan_tmp, b->mtile_sp, b->mtile_start, b->mtile_cnt};
+                  b->atom_m, b->cell_scan_tmp, b->mtile_sp, b->mtile_start, b->mtile_cnt,
+                  b->pnj, b->pnimg, b->pnd};
   for (void* p : ptrs) dfree(p);
   if (b->ctx->last_train_batch == b) b->ctx->last_train_batch = nullptr;
   delete b;

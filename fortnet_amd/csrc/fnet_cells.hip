@@ -77,7 +77,8 @@ struct CellArgs {
   const double* lat;
   double rc;
   long long a0, a1;
-  int K;  // sort capacity per wave (power of two >= largest row)
+  int K;  // sort capacity per wave (power of two >= largest row, or >= cap)
+  int cap;  // padded rows: entries per atom
   int* cnt;
   const int* rowptr;
   int* nj;
@@ -85,9 +86,12 @@ struct CellArgs {
   void* nd;
 };
 
-template <bool WRITE, typename V>
+// MODE: 0 writes the row lengths; 1 writes the sorted rows at their CSR offsets; 2 writes row i
+// at i * cap (entries past cap dropped) and its full length, in one pass.
+template <int MODE, typename V>
 __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
 #pragma clang fp contract(off)
+  constexpr bool WRITE = MODE != 0;
   extern __shared__ __align__(32) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -251,9 +255,11 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
       wave_sync();
     }
   }
-  const int rb = a.rowptr[i];
+  const long long rb = MODE == 2 ? i * a.cap : (long long)a.rowptr[i];
+  const int mw = MODE == 2 ? (m < a.cap ? m : a.cap) : m;
+  if (MODE == 2 && lane == 0) a.cnt[i] = count;
   V* out = (V*)a.nd;
-  for (int e = lane; e < m; e += 64) {
+  for (int e = lane; e < mw; e += 64) {
     const int q = slot[e];
     a.nj[rb + e] = pj[q];
     a.nimg[rb + e] = pimg[q];
@@ -261,8 +267,9 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
   }
 }
 
-template <bool WRITE, typename V>
+template <int MODE, typename V>
 hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s) {
+  constexpr bool WRITE = MODE != 0;
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
   CellArgs a;
@@ -279,14 +286,16 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
   a.rc = rc;
   a.a0 = a0;
   a.a1 = a1;
+  const int need = MODE == 2 ? b->nb_cap : b->max_nb;
   int K = 64;
-  while (K < b->max_nb) K <<= 1;
+  while (K < need) K <<= 1;
   a.K = K;
+  a.cap = b->nb_cap;
   a.cnt = b->cnt;
   a.rowptr = b->rowptr;
-  a.nj = b->nj;
-  a.nimg = b->nimg;
-  a.nd = b->nd;
+  a.nj = MODE == 2 ? b->pnj : b->nj;
+  a.nimg = MODE == 2 ? b->pnimg : b->nimg;
+  a.nd = MODE == 2 ? b->pnd : b->nd;
   size_t lds = 0;
   int wpb = 4;
   if (WRITE) {
@@ -296,7 +305,7 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
     lds = per * wpb;
   }
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
-  k_neigh_cells<WRITE, V><<<grid, 64 * wpb, lds, s>>>(a);
+  k_neigh_cells<MODE, V><<<grid, 64 * wpb, lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -319,14 +328,20 @@ hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s) {
 hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1,
                               hipStream_t s) {
   if (b->ncells == 0) return hipSuccess;
-  return launch_cells_t<false, double4>(b, rc, a0, a1, s);
+  return launch_cells_t<0, double4>(b, rc, a0, a1, s);
 }
 
 hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
                              hipStream_t s) {
   if (b->ncells == 0) return hipSuccess;
-  return fp64 ? launch_cells_t<true, double4>(b, rc, a0, a1, s)
-              : launch_cells_t<true, float4>(b, rc, a0, a1, s);
+  return fp64 ? launch_cells_t<1, double4>(b, rc, a0, a1, s)
+              : launch_cells_t<1, float4>(b, rc, a0, a1, s);
+}
+
+hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s) {
+  if (b->ncells == 0) return hipSuccess;
+  return fp64 ? launch_cells_t<2, double4>(b, rc, 0, b->N, s)
+              : launch_cells_t<2, float4>(b, rc, 0, b->N, s);
 }
 
 }  // namespace fnet

@@ -174,6 +174,13 @@ struct fnet_batch {
   long long npairs = 0;
   int max_nb = 0;
   int* d_maxnb = nullptr;
+  // one-pass search (batched launches): row i at i * nb_cap, then compacted into the CSR list;
+  // nb_cap comes from the previous list (0: count pass + fill pass)
+  int nb_cap = 0;
+  long long pad_len = 0;       // allocated padded entries
+  int* pnj = nullptr;
+  int* pnimg = nullptr;
+  void* pnd = nullptr;         // double4 (room for either precision)
   // cell lists of large periodic structures
   int4* cellinfo = nullptr;    // [S] bins per axis (x, y, z) and use flag (w)
   int* cell_off = nullptr;     // [S+1] first global cell of each structure
@@ -207,6 +214,9 @@ hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s);
 hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
                              hipStream_t s);
+hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s);
+hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s);
+hipError_t launch_compact_rows(fnet_batch* b, int fp64, hipStream_t s);
 hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s,
                        int* maxout);
 long long scan_tmp_size(long long n);

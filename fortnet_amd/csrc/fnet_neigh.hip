@@ -132,15 +132,20 @@ __device__ __forceinline__ bool near_test(double ci0, double ci1, double ci2, co
   return s <= rc2;
 }
 
-template <bool WRITE, typename V>
+// MODE: kCount writes the row lengths; kFill writes the rows at their CSR offsets; kPadded
+// writes row i at i * cap (entries past cap dropped) and its full length, in one pass.
+enum { kCount = 0, kFill = 1, kPadded = 2 };
+
+template <int MODE, typename V>
 __global__ void __launch_bounds__(256)
     k_neigh(const double* __restrict__ xyz, const long long* __restrict__ off,
             const int* __restrict__ atom_struct, const StructInfo* __restrict__ sinfo,
             const double* __restrict__ lat, double rc, long long a0, long long a1,
             int* __restrict__ cnt, const int* __restrict__ rowptr,
             int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd,
-            const int4* __restrict__ cellinfo) {
+            const int4* __restrict__ cellinfo, int cap) {
 #pragma clang fp contract(off)
+  constexpr bool WRITE = MODE != kCount;
   const int lane = threadIdx.x & 63;
   const long long i = a0 + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   if (i >= a1) return;
@@ -152,13 +157,14 @@ __global__ void __launch_bounds__(256)
   const double* X = xyz + 3 * o;
   const double ci0 = X[3 * li], ci1 = X[3 * li + 1], ci2 = X[3 * li + 2];
   const double rc2 = rc * rc;
-  const long long base = WRITE ? rowptr[i] : 0;
+  const long long base = MODE == kFill ? rowptr[i] : (MODE == kPadded ? i * cap : 0);
   int count = 0;
 
   auto emit = [&](bool hit, int j, int img, double nd0, double nd1, double nd2, double d2) {
     const unsigned long long mask = __ballot(hit);
-    if (WRITE && hit) {
-      const long long pos = base + count + __popcll(mask & lanemask_lt(lane));
+    const int q = count + __popcll(mask & lanemask_lt(lane));
+    if (WRITE && hit && (MODE != kPadded || q < cap)) {
+      const long long pos = base + q;
       nj[pos] = (int)(o + j);
       nimg[pos] = img;
       V v;
@@ -256,7 +262,24 @@ __global__ void __launch_bounds__(256)
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
-  if (!WRITE && lane == 0) cnt[i] = count;
+  if (MODE != kFill && lane == 0) cnt[i] = count;
+}
+
+// Copies the padded rows (cap entries per atom) to their CSR offsets.
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_compact_rows(long long n, int cap, const int* __restrict__ cnt,
+                   const int* __restrict__ rowptr, const int* __restrict__ pnj,
+                   const int* __restrict__ pnimg, const V* __restrict__ pnd,
+                   int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i = t / cap;
+  const int e = (int)(t - i * cap);
+  if (i >= n || e >= cnt[i]) return;
+  const long long src = t, dst = (long long)rowptr[i] + e;
+  nj[dst] = pnj[src];
+  nimg[dst] = pnimg[src];
+  nd[dst] = pnd[src];
 }
 
 // ---- exclusive scan of the per-atom counts -------------------------------------------------
@@ -363,9 +386,9 @@ hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long 
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((n + 3) / 4);
-  k_neigh<false, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
-                                               rc, a0, a1, b->cnt, nullptr, nullptr, nullptr,
-                                               nullptr, b->cellinfo);
+  k_neigh<kCount, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
+                                                 b->lat, rc, a0, a1, b->cnt, nullptr, nullptr,
+                                                 nullptr, nullptr, b->cellinfo, 0);
   return hipGetLastError();
 }
 
@@ -375,13 +398,47 @@ hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a
   if (n <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((n + 3) / 4);
   if (fp64) {
-    k_neigh<true, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
-                                                rc, a0, a1, nullptr, b->rowptr, b->nj, b->nimg,
-                                                (double4*)b->nd, b->cellinfo);
+    k_neigh<kFill, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
+                                                 b->lat, rc, a0, a1, nullptr, b->rowptr, b->nj,
+                                                 b->nimg, (double4*)b->nd, b->cellinfo, 0);
   } else {
-    k_neigh<true, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo, b->lat,
-                                               rc, a0, a1, nullptr, b->rowptr, b->nj, b->nimg,
-                                               (float4*)b->nd, b->cellinfo);
+    k_neigh<kFill, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
+                                                b->lat, rc, a0, a1, nullptr, b->rowptr, b->nj,
+                                                b->nimg, (float4*)b->nd, b->cellinfo, 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s) {
+  const long long n = b->N;
+  if (n <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((n + 3) / 4);
+  if (fp64) {
+    k_neigh<kPadded, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
+                                                   b->lat, rc, 0, n, b->cnt, nullptr, b->pnj,
+                                                   b->pnimg, (double4*)b->pnd, b->cellinfo,
+                                                   b->nb_cap);
+  } else {
+    k_neigh<kPadded, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
+                                                  b->lat, rc, 0, n, b->cnt, nullptr, b->pnj,
+                                                  b->pnimg, (float4*)b->pnd, b->cellinfo,
+                                                  b->nb_cap);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_compact_rows(fnet_batch* b, int fp64, hipStream_t s) {
+  const long long tot = (long long)b->N * b->nb_cap;
+  if (tot <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((tot + 255) / 256);
+  if (fp64) {
+    k_compact_rows<double4><<<grid, 256, 0, s>>>(b->N, b->nb_cap, b->cnt, b->rowptr, b->pnj,
+                                                 b->pnimg, (const double4*)b->pnd, b->nj,
+                                                 b->nimg, (double4*)b->nd);
+  } else {
+    k_compact_rows<float4><<<grid, 256, 0, s>>>(b->N, b->nb_cap, b->cnt, b->rowptr, b->pnj,
+                                                b->pnimg, (const float4*)b->pnd, b->nj, b->nimg,
+                                                (float4*)b->nd);
   }
   return hipGetLastError();
 }

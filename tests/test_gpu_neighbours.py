@@ -32,23 +32,27 @@ def _model(rc, species_z=(14,)):
     return fa.Netstat(g, species_z, dims, 0, np.zeros((nsp, fa.nparams(dims))))
 
 
+def _check_rows(geos, rc, rp, j, img):
+    off = 0
+    for g in geos:
+        st = O.Structure(g.coords, g.latvecs, g.periodic, g.atnum)
+        for i in range(g.n):
+            oj, oimg, _, _ = O.neighbours(st, rc, i)
+            a = off + i
+            gj = j[rp[a]:rp[a + 1]] - off
+            gi = img[rp[a]:rp[a + 1]]
+            assert np.array_equal(gj, oj), (i, gj[:10], oj[:10])
+            assert np.array_equal(gi, oimg), i
+        off += g.n
+
+
 def _compare(ctx, geos, rc):
     ctx.upload_model(_model(rc, sorted(set(int(z) for g in geos for z in g.atnum))))
     for prec in (fa.FNET_FP64, fa.FNET_FP32):
         b = ctx.batch(geos, prec)
         P = b.neighbour_list()
         rp, j, img = b.neighbours()
-        off = 0
-        for g in geos:
-            st = O.Structure(g.coords, g.latvecs, g.periodic, g.atnum)
-            for i in range(g.n):
-                oj, oimg, _, _ = O.neighbours(st, rc, i)
-                a = off + i
-                gj = j[rp[a]:rp[a + 1]] - off
-                gi = img[rp[a]:rp[a + 1]]
-                assert np.array_equal(gj, oj), (i, gj[:10], oj[:10])
-                assert np.array_equal(gi, oimg), i
-            off += g.n
+        _check_rows(geos, rc, rp, j, img)
         assert P == rp[-1]
         b.close()
 
@@ -94,6 +98,34 @@ def test_cell_list_mid_size_cells(ctx):
     rng = np.random.default_rng(3)
     geos = [workloads.random_cell(rng, n, 2.33, 2.0, 14) for n in (300, 420, 512)]
     _compare(ctx, geos, 6.0 * fa.AA__BOHR)
+
+
+@pytest.mark.parametrize("precision", [fa.FNET_FP64, fa.FNET_FP32], ids=["fp64", "fp32"])
+def test_one_pass_search_and_overflow(ctx, precision):
+    """After the first list of a batch, batched lists come from one search pass into rows sized by
+    the previous list, compacted afterwards.  Compressing the cells by 15 % gives rows longer than
+    that: the batch must fall back to the counted fill, and every list must be the reference's."""
+    rng = np.random.default_rng(5)
+    geos = [workloads.random_cell(rng, n, 2.33, 2.0, 14) for n in (13, 40, 300)]
+    geos.append(W.si_diamond(4, 4, 4, 0.1, 2))  # 512 atoms, cell-list search
+    rc = 6.0 * fa.AA__BOHR
+    ctx.upload_model(_model(rc))
+    b = ctx.batch(geos, precision)
+    b.neighbour_list()
+    first = b.neighbours()
+    b.neighbour_list()
+    second = b.neighbours()
+    for x, y in zip(first, second):
+        assert np.array_equal(x, y)
+    _check_rows(geos, rc, *second)
+    dense = [fa.Geometry(g.coords * 0.85, g.atnum, g.species, g.latvecs * 0.85) for g in geos]
+    b.update_coords(np.concatenate([g.coords for g in dense]), np.stack([g.latvecs for g in dense]))
+    for _ in range(2):  # overflow, then one pass with the enlarged rows
+        P = b.neighbour_list()
+        rp, j, img = b.neighbours()
+        assert P == rp[-1]
+        _check_rows(dense, rc, rp, j, img)
+    b.close()
 
 
 def test_triclinic_and_unwrapped(ctx):
