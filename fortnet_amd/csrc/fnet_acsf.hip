@@ -86,7 +86,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 struct AcsfArgs {
-  const int* rowptr;
+  const int* rowptr;  // CSR rows, or (cap > 0) row i at i * cap with cnt[i] entries
+  const int* cnt;
+  int cap;
   const int* nj;
   const void* nd;
   const int* zcode;
@@ -618,8 +620,8 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   const int M = a.maxnb;
   Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M) * wave, M);
 
-  const int b = a.rowptr[i];
-  const int n = a.rowptr[i + 1] - b;
+  const long long b = a.cap ? i * a.cap : (long long)a.rowptr[i];
+  const int n = a.cap ? a.cnt[i] : (int)(a.rowptr[i + 1] - b);
   const V* nd = (const V*)a.nd + b;
   const T pi = (T)kPi;
   const T rcl = (T)a.rclist;
@@ -981,8 +983,10 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   if (sb * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
   AcsfArgs a;
   a.rowptr = b->rowptr;
-  a.nj = b->nj;
-  a.nd = b->nd;
+  a.cnt = b->cnt;
+  a.cap = b->nl_padded ? b->nb_cap : 0;
+  a.nj = b->nl_padded ? b->pnj : b->nj;
+  a.nd = b->nl_padded ? b->pnd : b->nd;
   a.zcode = b->zcode;
   a.ext = b->ext;
   a.next = b->next;

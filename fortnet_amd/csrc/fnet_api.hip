@@ -304,8 +304,9 @@ int do_neighbours(fnet_batch* b) {
     b->pcap = cap;
   }
   b->npairs = P;
-  if (padded && b->max_nb <= b->nb_cap) {
-    HIPCHECK(ctx, launch_compact_rows(b, fp64, s));
+  b->nl_padded = padded && b->max_nb <= b->nb_cap;
+  if (b->nl_padded) {
+    // the ACSF kernels read the padded rows; fnet_get_neighbours compacts them
   } else if (b->launch == FNET_PER_STRUCT) {
     for (int st = 0; st < b->S; ++st) {
       HIPCHECK(ctx, launch_neigh_fill(b, rc, b->h_off[st], b->h_off[st + 1], fp64, s));
@@ -947,6 +948,8 @@ int fnet_get_neighbours(fnet_ctx* ctx, fnet_batch* b, long long* rowptr, int* j,
   int rc = check_ready(ctx, b);
   if (rc) return rc;
   if (!b->have_nl) return fail(ctx, FNET_ERR_STATE, "no neighbour list computed");
+  if (b->nl_padded)
+    HIPCHECK(ctx, launch_compact_rows(b, b->result_precision == FNET_FP64, ctx->stream));
   std::vector<int> rp(b->N + 1);
   HIPCHECK(ctx, hipMemcpyAsync(rp.data(), b->rowptr, sizeof(int) * (b->N + 1),
                                hipMemcpyDeviceToHost, ctx->stream));

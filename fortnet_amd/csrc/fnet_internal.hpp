@@ -177,6 +177,7 @@ struct fnet_batch {
   // one-pass search (batched launches): row i at i * nb_cap, then compacted into the CSR list;
   // nb_cap comes from the previous list (0: count pass + fill pass)
   int nb_cap = 0;
+  bool nl_padded = false;      // the current list is the padded rows (compacted on download)
   long long pad_len = 0;       // allocated padded entries
   int* pnj = nullptr;
   int* pnimg = nullptr;

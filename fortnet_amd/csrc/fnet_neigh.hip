@@ -57,10 +57,13 @@ __device__ void lattice_setup(const double* L, double rc, StructInfo& si) {
 }
 
 // One block per structure: lattice set-up by thread 0, extent of fractional coordinates by all.
+// The extent only serves the image pruning of the direct search; structures searched with cell
+// lists skip it.
 __global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ xyz,
                                                      const double* __restrict__ lat,
                                                      const int* __restrict__ periodic,
                                                      const long long* __restrict__ off, double rc,
+                                                     const int4* __restrict__ cellinfo,
                                                      StructInfo* __restrict__ sinfo) {
   __shared__ StructInfo si;
   __shared__ double red[2][3][256];
@@ -81,7 +84,7 @@ __global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ 
   }
   __syncthreads();
   double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-  if (si.periodic) {
+  if (si.periodic && !cellinfo[s].w) {
     const long long a0 = off[s], a1 = off[s + 1];
     for (long long i = a0 + tid; i < a1; i += blockDim.x) {
       const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
@@ -377,7 +380,8 @@ hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t
 
 hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s) {
   if (b->S == 0) return hipSuccess;
-  k_struct_prep<<<b->S, 256, 0, s>>>(b->xyz, b->lat, b->periodic, b->off, rc, b->sinfo);
+  k_struct_prep<<<b->S, 256, 0, s>>>(b->xyz, b->lat, b->periodic, b->off, rc, b->cellinfo,
+                                     b->sinfo);
   return hipGetLastError();
 }
 
