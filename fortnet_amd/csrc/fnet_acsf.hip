@@ -79,6 +79,18 @@ __device__ __forceinline__ double cutarg(double r, double pi, double rc, double 
   return pi * r / rc;
 }
 
+// A wave-uniform value moved to scalar registers (values converted from FP64 on the vector
+// unit would otherwise hold one vector register each through the pair loops).
+__device__ __forceinline__ float uniform(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniform(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -248,6 +260,76 @@ struct PairWalk {
   }
 };
 
+// The walk of PairWalk with two offsets per step: lane pairs (j, d) with d = dmin + 2e and d + 1,
+// k = (j + d) mod LB, so one step covers two pairs that share j (its loads, and in the adjoint
+// its accumulator update) and the 64 lanes still touch 64 consecutive entries.  v1 / v2 say
+// whether each pair exists (the last offset of a row may not); an invalid pair's k is still a
+// valid entry index.
+template <bool SAME>
+struct PairWalk2 {
+  int A0, B0, LA, LB;
+  int j, e, k1, u, uend;
+  int rs, qs, kstep, kwrap, dfull, dhalf;
+
+  __device__ __forceinline__ void init(int lane, int a0, int l1, int b0, int l2) {
+    if (!SAME && l2 > l1) {
+      A0 = b0;
+      LA = l2;
+      B0 = a0;
+      LB = l1;
+    } else {
+      A0 = a0;
+      LA = l1;
+      B0 = b0;
+      LB = l2;
+    }
+    j = e = k1 = rs = qs = kstep = kwrap = dfull = dhalf = 0;
+    u = lane;
+    uend = 0;
+    if (LA <= 0 || LB <= 0) return;
+    int dtot;
+    if (SAME) {  // unordered pairs: d = 1 .. (LA - 1) / 2, and d = LA / 2 for j < LA / 2
+      dfull = (LA - 1) / 2;
+      dhalf = (LA & 1) == 0 ? LA / 2 : 0;
+      dtot = dfull + (dhalf ? 1 : 0);
+    } else {  // every (j, k): d = 0 .. LB - 1
+      dfull = LB - 1;
+      dtot = LB;
+    }
+    uend = LA * ((dtot + 1) / 2);
+    j = lane % LA;
+    e = lane / LA;
+    k1 = (j + (SAME ? 1 : 0) + 2 * e) % LB;
+    qs = 64 / LA;
+    rs = 64 % LA;
+    kstep = (rs + 2 * qs) % LB;
+    kwrap = ((2 - LA) % LB + LB) % LB;
+  }
+
+  __device__ __forceinline__ bool next(int& ej, int& ek1, int& ek2, bool& v1, bool& v2) {
+    if (u >= uend) return false;
+    const int d1 = (SAME ? 1 : 0) + 2 * e;
+    v1 = d1 <= dfull || (SAME && d1 == dhalf && j < dhalf);
+    v2 = d1 + 1 <= dfull || (SAME && d1 + 1 == dhalf && j < dhalf);
+    const int k2 = k1 + 1 >= LB ? k1 + 1 - LB : k1 + 1;
+    ej = A0 + j;
+    ek1 = B0 + k1;
+    ek2 = B0 + k2;
+    u += 64;
+    j += rs;
+    e += qs;
+    k1 += kstep;
+    k1 = (k1 >= LB) ? k1 - LB : k1;
+    if (j >= LA) {
+      j -= LA;
+      e += 1;
+      k1 += kwrap;
+      k1 = (k1 >= LB) ? k1 - LB : k1;
+    }
+    return true;
+  }
+};
+
 // x^n, n >= 0 wave-uniform: the common exponents of ACSF sets as straight-line products (the
 // switch is on a scalar, so only the taken product costs vector instructions), others by square
 // and multiply
@@ -377,20 +459,28 @@ __device__ __forceinline__ bool adj_geom(const typename Vec4<T>::type& dj,
 // with t = S1 E_j E_k H and h = S0 E_j E_k H' / r_jk (the derivative of
 // S0 * E_j E_k H through c, r_j, r_k and r_jk; gFuncGrad, acsf.F90:1496-1670).
 template <typename T, bool G4>
+__device__ __forceinline__ void adj_coef(const typename Vec4<T>::type& pj,
+                                         const typename Vec4<T>::type& pk, T rr, T c, T H, T Hq,
+                                         T S0, T S1, T& Pj, T& Pk, T& Q) {
+  const T EE = pj.y * pk.y;
+  const T t = S1 * EE * H;
+  const T U = S0 * H;
+  const T h = G4 ? S0 * EE * Hq : (T)0;
+  Q = t * rr - h;
+  const T ct = c * t;
+  Pj = U * pk.y * pj.w + h - ct * (pj.x * pj.x);
+  Pk = U * pj.y * pk.w + h - ct * (pk.x * pk.x);
+}
+
+template <typename T, bool G4>
 __device__ __forceinline__ void adj_scatter(const Slice<T>& sl, int ej, int ek,
                                             const typename Vec4<T>::type& dj,
                                             const typename Vec4<T>::type& dk,
                                             const typename Vec4<T>::type& pj,
                                             const typename Vec4<T>::type& pk, T rr, T c, T H, T Hq,
                                             T S0, T S1) {
-  const T EE = pj.y * pk.y;
-  const T t = S1 * EE * H;
-  const T U = S0 * H;
-  const T h = G4 ? S0 * EE * Hq : (T)0;
-  const T Q = t * rr - h;
-  const T ct = c * t;
-  const T Pj = U * pk.y * pj.w + h - ct * (pj.x * pj.x);
-  const T Pk = U * pj.y * pk.w + h - ct * (pk.x * pk.x);
+  T Pj, Pk, Q;
+  adj_coef<T, G4>(pj, pk, rr, c, H, Hq, S0, S1, Pj, Pk, Q);
   lds_f64* gj = sl.gs + __umul24(ej, 3);
   lds_f64* gk = sl.gs + __umul24(ek, 3);
   lds_add(gj + 0, (double)(Pj * dj.x + Q * dk.x));
@@ -458,44 +548,58 @@ __device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int l
 // P^(2^m) for every slot and, as the running product, P^(2^m - 1) for the derivative: the same
 // straight-line code for every slot, with no per-member branch on the exponent.
 
-// Forward: off-diagonal pairs, FP64 partial sums per slot.
+// Forward: off-diagonal pairs, two per step (PairWalk2), FP64 partial sums per slot; the two
+// pairs' terms are added in T before the conversion.
 template <typename T, int MX, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, T rc, T eta, T pi, T pirc,
                                               double (&acc)[2][MX + 1]) {
   using V = typename Vec4<T>::type;
-  PairWalk<SAME> it;
+  PairWalk2<SAME> it;
   it.init(lane, A0, LA, B0, LB);
-  int ej, ek;
-  while (it.next(ej, ek)) {
-    const V dj = sl.d[ej], dk = sl.d[ek];
-    const V pj = sl.pa[ej], pk = sl.pa[ek];
-    T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
+  int ej, ek1, ek2;
+  bool v1, v2;
+  // radial-and-r_jk factor of one pair (0 when it does not exist or r_jk > rc) and its cos(theta)
+  auto pair_base = [&](const V& dj, const V& pj, const V& dk, const V& pk, bool v, T& c) {
+    c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
     c = fmin(fmax(c, (T)-1), (T)1);
     T base = (SAME ? (T)2 : (T)1) * pj.y * pk.y;
+    bool in = v;
     if (G4) {
       const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
       const T r2 = rx * rx + ry * ry + rz * rz;
       const T rjk = fsqrt(r2);
       const T fcjk = (T)0.5 * pj.z * pk.z * (fcos(cutarg(rjk, pi, rc, pirc)) + (T)1);
-      const T f = fexp(-eta * r2) * fcjk;
-      base = (rjk <= rc) ? base * f : (T)0;
+      base *= fexp(-eta * r2) * fcjk;
+      in = in && (rjk <= rc);
     }
+    return in ? base : (T)0;
+  };
+  while (it.next(ej, ek1, ek2, v1, v2)) {
+    const V dj = sl.d[ej], pj = sl.pa[ej];
+    const V dk1 = sl.d[ek1], pk1 = sl.pa[ek1];
+    const V dk2 = sl.d[ek2], pk2 = sl.pa[ek2];
+    T c1, c2;
+    const T b1 = pair_base(dj, pj, dk1, pk1, v1, c1);
+    const T b2 = pair_base(dj, pj, dk2, pk2, v2, c2);
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-      T p = (T)1 + (l ? lam1 : lam0) * c;
-      acc[l][0] += (double)(base * p);
+      const T lam = l ? lam1 : lam0;
+      T p1 = (T)1 + lam * c1, p2 = (T)1 + lam * c2;
+      acc[l][0] += (double)(b1 * p1 + b2 * p2);
 #pragma unroll
       for (int m = 1; m <= MX; ++m) {
-        p = p * p;
-        acc[l][m] += (double)(base * p);
+        p1 = p1 * p1;
+        p2 = p2 * p2;
+        acc[l][m] += (double)(b1 * p1 + b2 * p2);
       }
     }
   }
 }
 
-// Adjoint: off-diagonal pairs; c0 = w dE/dG 2^(1 - xi) of the slot's member (0 for an empty
-// slot, w the pair weight), c1 = c0 xi lambda; c1const = the c1 of the xi = 1 slots, whose
+// Adjoint: off-diagonal pairs, two per step (PairWalk2) sharing j, whose gradient is summed in
+// registers and added once.  c0 = w dE/dG 2^(1 - xi) of the slot's member (0 for an empty slot,
+// w the pair weight), c1 = c0 xi lambda; c1const = the c1 of the xi = 1 slots, whose
 // P^(xi - 1) is 1.
 template <typename T, int MX, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
@@ -503,14 +607,15 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
                                               const T (&c1)[2][MX + 1], T c1const, T rc, T eta,
                                               T pi, T pirc) {
   using V = typename Vec4<T>::type;
-  PairWalk<SAME> it;
+  PairWalk2<SAME> it;
   it.init(lane, A0, LA, B0, LB);
-  int ej, ek;
-  while (it.next(ej, ek)) {
-    const V dj = sl.d[ej], dk = sl.d[ek];
-    const V pj = sl.pa[ej], pk = sl.pa[ek];
+  int ej, ek1, ek2;
+  bool v1, v2;
+  // one pair: adds its k-side gradient to the accumulator of ek, returns the j-side in g
+  auto pair = [&](const V& dj, const V& pj, int ek, bool v, T& gx, T& gy, T& gz) {
+    const V dk = sl.d[ek], pk = sl.pa[ek];
     T rr, c, H, Hq;
-    if (!adj_geom<T, G4>(dj, dk, pj, pk, rc, eta, pi, pirc, rr, c, H, Hq)) continue;
+    if (!v || !adj_geom<T, G4>(dj, dk, pj, pk, rc, eta, pi, pirc, rr, c, H, Hq)) return;
     T S0 = 0, S1 = c1const;
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
@@ -525,7 +630,25 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
         S1 += c1[l][m] * pm;
       }
     }
-    adj_scatter<T, G4>(sl, ej, ek, dj, dk, pj, pk, rr, c, H, Hq, S0, S1);
+    T Pj, Pk, Q;
+    adj_coef<T, G4>(pj, pk, rr, c, H, Hq, S0, S1, Pj, Pk, Q);
+    gx += Pj * dj.x + Q * dk.x;
+    gy += Pj * dj.y + Q * dk.y;
+    gz += Pj * dj.z + Q * dk.z;
+    lds_f64* gk = sl.gs + __umul24(ek, 3);
+    lds_add(gk + 0, (double)(Pk * dk.x + Q * dj.x));
+    lds_add(gk + 1, (double)(Pk * dk.y + Q * dj.y));
+    lds_add(gk + 2, (double)(Pk * dk.z + Q * dj.z));
+  };
+  while (it.next(ej, ek1, ek2, v1, v2)) {
+    const V dj = sl.d[ej], pj = sl.pa[ej];
+    T gx = 0, gy = 0, gz = 0;
+    pair(dj, pj, ek1, v1, gx, gy, gz);
+    pair(dj, pj, ek2, v2, gx, gy, gz);
+    lds_f64* gj = sl.gs + __umul24(ej, 3);
+    lds_add(gj + 0, (double)gx);
+    lds_add(gj + 1, (double)gy);
+    lds_add(gj + 2, (double)gz);
   }
 }
 
@@ -537,7 +660,14 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
   using V = typename Vec4<T>::type;
   const bool g4 = grp.type == 4;
   const bool same = grp.same != 0;
-  const T lam0 = (T)grp.lamv[0], lam1 = (T)grp.lamv[1];
+  T lam0 = (T)grp.lamv[0], lam1 = (T)grp.lamv[1];
+  if (ADJ) {
+    lam0 = uniform(lam0);
+    lam1 = uniform(lam1);
+    rc = uniform(rc);
+    eta = uniform(eta);
+    pirc = uniform(pirc);
+  }
   if (!ADJ) {
     double acc[2][MX + 1];
 #pragma unroll
@@ -581,11 +711,12 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
       for (int m = 0; m <= MX; ++m) {
         const int q = grp.smem[5 * l + m];
         const T v = q >= 0 ? (T)(cG[grp.col[q]] * grp.scale[q]) : (T)0;
-        c0[l][m] = w * v;
-        c1[l][m] = w * v * (T)(1 << m) * (l ? lam1 : lam0);
+        c0[l][m] = uniform(w * v);
+        c1[l][m] = uniform(w * v * (T)(1 << m) * (l ? lam1 : lam0));
         S0d += v * (T)grp.lpdiag[5 * l + m];
       }
-    const T c1const = c1[0][0] + c1[1][0];
+    S0d = uniform(S0d);
+    const T c1const = uniform(c1[0][0] + c1[1][0]);
     if (same) {
       // j == k terms: only the radial factors depend on r_j; P = 1 + lambda at c = 1
       // (acsf.F90:1580-1585).  Each lane owns its entries here, so the update is plain.
