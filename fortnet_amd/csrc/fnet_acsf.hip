@@ -451,6 +451,72 @@ __device__ __forceinline__ bool adj_geom(const typename Vec4<T>::type& dj,
   return true;
 }
 
+// Wave-uniform constants of a group's r_jk factor exp(-eta r^2) fc(r).  FP32: -eta log2(e) and
+// pi / (2 pi rc) prepared for v_exp_f32 (2^x) and v_cos_f32 / v_sin_f32 (argument in turns);
+// FP64: libm in the reference's operation order.
+template <typename T> struct JkConst;
+template <> struct JkConst<float> {
+  float rc, eta, pirc, el2, prev;
+  __device__ __forceinline__ void set(float rc_, float eta_, float pirc_) {
+    rc = rc_;
+    eta = eta_;
+    pirc = pirc_;
+    el2 = uniform(-eta_ * 1.44269504088896341f);
+    prev = uniform(pirc_ * 0.159154943091895336f);
+  }
+  __device__ __forceinline__ float ex(float r2) const { return __builtin_amdgcn_exp2f(r2 * el2); }
+  __device__ __forceinline__ void cs(float r, float& c, float& sn) const {
+    const float x = r * prev;
+    c = __builtin_amdgcn_cosf(x);
+    sn = __builtin_amdgcn_sinf(x);
+  }
+};
+template <> struct JkConst<double> {
+  double rc, eta, pirc;
+  __device__ __forceinline__ void set(double rc_, double eta_, double pirc_) {
+    rc = rc_;
+    eta = eta_;
+    pirc = pirc_;
+  }
+  __device__ __forceinline__ double ex(double r2) const { return exp(-eta * r2); }
+  __device__ __forceinline__ void cs(double r, double& c, double& sn) const {
+    const double x = kPi * r / rc;
+    c = cos(x);
+    sn = sin(x);
+  }
+};
+
+// adj_geom without branches: a pair that does not exist (v false) or lies beyond the cutoff
+// gets H = Hq = 0 (and so no contribution) by selection, which also keeps the values of a
+// degenerate invalid pair (r_jk = 0) out of the result.
+template <typename T, bool G4>
+__device__ __forceinline__ void adj_geom_sel(const typename Vec4<T>::type& dj,
+                                             const typename Vec4<T>::type& dk,
+                                             const typename Vec4<T>::type& pj,
+                                             const typename Vec4<T>::type& pk,
+                                             const JkConst<T>& K, bool v, T& rr, T& c, T& H,
+                                             T& Hq) {
+  rr = pj.x * pk.x;
+  c = fmin(fmax((dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * rr, (T)-1), (T)1);
+  H = v ? (T)1 : (T)0;
+  Hq = 0;
+  if (G4) {
+    const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
+    const T r2 = rx * rx + ry * ry + rz * rz;
+    T rjk, irjk;
+    dist_inv(r2, rjk, irjk);
+    const T qq = pj.z * pk.z;
+    T cx, sx;
+    K.cs(rjk, cx, sx);
+    const T ex = K.ex(r2);
+    const T fc = (T)0.5 * qq * (cx + (T)1);
+    const T dfc = -(T)0.5 * qq * K.pirc * sx;
+    const bool ok = v && (rjk <= K.rc);
+    H = ok ? ex * fc : (T)0;
+    Hq = ok ? ex * (dfc - (T)2 * K.eta * rjk * fc) * irjk : (T)0;
+  }
+}
+
 // Adds the gradient of one pair to the per-entry accumulators.  S0 = sum over members of
 // dE/dG (1 + lambda c)^xi and S1 = its derivative in c, both times the pair weight.  With the
 // per-entry factors E = exp(-eta r^2) fc(r) (pa.y) and E' / r (pa.w), the gradients are
@@ -552,7 +618,7 @@ __device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int l
 // pairs' terms are added in T before the conversion.
 template <typename T, int MX, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
-                                              int LB, T lam0, T lam1, T rc, T eta, T pi, T pirc,
+                                              int LB, T lam0, T lam1, const JkConst<T>& K,
                                               double (&acc)[2][MX + 1]) {
   using V = typename Vec4<T>::type;
   PairWalk2<SAME> it;
@@ -569,9 +635,11 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
       const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
       const T r2 = rx * rx + ry * ry + rz * rz;
       const T rjk = fsqrt(r2);
-      const T fcjk = (T)0.5 * pj.z * pk.z * (fcos(cutarg(rjk, pi, rc, pirc)) + (T)1);
-      base *= fexp(-eta * r2) * fcjk;
-      in = in && (rjk <= rc);
+      T cx, sx;
+      K.cs(rjk, cx, sx);
+      const T fcjk = (T)0.5 * pj.z * pk.z * (cx + (T)1);
+      base *= K.ex(r2) * fcjk;
+      in = in && (rjk <= K.rc);
     }
     return in ? base : (T)0;
   };
@@ -604,8 +672,8 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
 template <typename T, int MX, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, const T (&c0)[2][MX + 1],
-                                              const T (&c1)[2][MX + 1], T c1const, T rc, T eta,
-                                              T pi, T pirc) {
+                                              const T (&c1)[2][MX + 1], T c1const,
+                                              const JkConst<T>& K) {
   using V = typename Vec4<T>::type;
   PairWalk2<SAME> it;
   it.init(lane, A0, LA, B0, LB);
@@ -615,7 +683,7 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
   auto pair = [&](const V& dj, const V& pj, int ek, bool v, T& gx, T& gy, T& gz) {
     const V dk = sl.d[ek], pk = sl.pa[ek];
     T rr, c, H, Hq;
-    if (!v || !adj_geom<T, G4>(dj, dk, pj, pk, rc, eta, pi, pirc, rr, c, H, Hq)) return;
+    adj_geom_sel<T, G4>(dj, dk, pj, pk, K, v, rr, c, H, Hq);
     T S0 = 0, S1 = c1const;
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
@@ -668,6 +736,8 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
     eta = uniform(eta);
     pirc = uniform(pirc);
   }
+  JkConst<T> K;
+  K.set(rc, eta, pirc);
   if (!ADJ) {
     double acc[2][MX + 1];
 #pragma unroll
@@ -685,11 +755,11 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
 #pragma unroll
           for (int m = 0; m <= MX; ++m) acc[l][m] += (double)(base * (T)grp.lpdiag[5 * l + m]);
       }
-      if (g4) pairs_fwd_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
-      else pairs_fwd_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
+      if (g4) pairs_fwd_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
+      else pairs_fwd_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
     } else {
-      if (g4) pairs_fwd_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
-      else pairs_fwd_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, rc, eta, pi, pirc, acc);
+      if (g4) pairs_fwd_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
+      else pairs_fwd_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
     }
 #pragma unroll
     for (int l = 0; l < 2; ++l)
@@ -730,11 +800,11 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
         sl.gs[3 * ej + 1] += gsc * dj.y;
         sl.gs[3 * ej + 2] += gsc * dj.z;
       }
-      if (g4) pairs_adj_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
-      else pairs_adj_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
+      if (g4) pairs_adj_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
+      else pairs_adj_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
     } else {
-      if (g4) pairs_adj_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
-      else pairs_adj_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, rc, eta, pi, pirc);
+      if (g4) pairs_adj_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
+      else pairs_adj_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
     }
   }
 }
