@@ -616,7 +616,7 @@ __device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int l
 
 // Forward: off-diagonal pairs, two per step (PairWalk2), FP64 partial sums per slot; the two
 // pairs' terms are added in T before the conversion.
-template <typename T, int MX, bool SAME, bool G4>
+template <typename T, int MX, int MASK, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, const JkConst<T>& K,
                                               double (&acc)[2][MX + 1]) {
@@ -654,12 +654,12 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
     for (int l = 0; l < 2; ++l) {
       const T lam = l ? lam1 : lam0;
       T p1 = (T)1 + lam * c1, p2 = (T)1 + lam * c2;
-      acc[l][0] += (double)(b1 * p1 + b2 * p2);
+      if (MASK & 1) acc[l][0] += (double)(b1 * p1 + b2 * p2);
 #pragma unroll
       for (int m = 1; m <= MX; ++m) {
         p1 = p1 * p1;
         p2 = p2 * p2;
-        acc[l][m] += (double)(b1 * p1 + b2 * p2);
+        if (MASK & (1 << m)) acc[l][m] += (double)(b1 * p1 + b2 * p2);
       }
     }
   }
@@ -669,7 +669,7 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
 // registers and added once.  c0 = w dE/dG 2^(1 - xi) of the slot's member (0 for an empty slot,
 // w the pair weight), c1 = c0 xi lambda; c1const = the c1 of the xi = 1 slots, whose
 // P^(xi - 1) is 1.
-template <typename T, int MX, bool SAME, bool G4>
+template <typename T, int MX, int MASK, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, const T (&c0)[2][MX + 1],
                                               const T (&c1)[2][MX + 1], T c1const,
@@ -689,13 +689,15 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
     for (int l = 0; l < 2; ++l) {
       T p = (T)1 + (l ? lam1 : lam0) * c;
       T pm = (T)1;
-      S0 += c0[l][0] * p;
+      if (MASK & 1) S0 += c0[l][0] * p;
 #pragma unroll
       for (int m = 1; m <= MX; ++m) {
         pm = pm * p;
         p = p * p;
-        S0 += c0[l][m] * p;
-        S1 += c1[l][m] * pm;
+        if (MASK & (1 << m)) {
+          S0 += c0[l][m] * p;
+          S1 += c1[l][m] * pm;
+        }
       }
     }
     T Pj, Pk, Q;
@@ -720,8 +722,9 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
   }
 }
 
-// One ladder group of one centre: values (forward) or the per-entry adjoint accumulation.
-template <typename T, bool ADJ, int MX>
+// One ladder group of one centre: values (forward) or the per-entry adjoint accumulation.  MASK:
+// the exponent classes m (xi = 2^m) some member of the group has; the others are not evaluated.
+template <typename T, bool ADJ, int MX, int MASK>
 __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, long long i, int A0,
                           int L1, int B0, int L2, T rc, T eta, T pi, T pirc, const double* cG,
                           double* G, int nG) {
@@ -755,11 +758,11 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
 #pragma unroll
           for (int m = 0; m <= MX; ++m) acc[l][m] += (double)(base * (T)grp.lpdiag[5 * l + m]);
       }
-      if (g4) pairs_fwd_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
-      else pairs_fwd_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
+      if (g4) pairs_fwd_lad<T, MX, MASK, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
+      else pairs_fwd_lad<T, MX, MASK, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
     } else {
-      if (g4) pairs_fwd_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
-      else pairs_fwd_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
+      if (g4) pairs_fwd_lad<T, MX, MASK, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
+      else pairs_fwd_lad<T, MX, MASK, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
     }
 #pragma unroll
     for (int l = 0; l < 2; ++l)
@@ -800,11 +803,11 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
         sl.gs[3 * ej + 1] += gsc * dj.y;
         sl.gs[3 * ej + 2] += gsc * dj.z;
       }
-      if (g4) pairs_adj_lad<T, MX, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
-      else pairs_adj_lad<T, MX, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
+      if (g4) pairs_adj_lad<T, MX, MASK, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
+      else pairs_adj_lad<T, MX, MASK, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
     } else {
-      if (g4) pairs_adj_lad<T, MX, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
-      else pairs_adj_lad<T, MX, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
+      if (g4) pairs_adj_lad<T, MX, MASK, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
+      else pairs_adj_lad<T, MX, MASK, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const, K);
     }
   }
 }
@@ -1051,10 +1054,17 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
 
       if constexpr (LAD) {
         switch (grp.lad) {
-          case 1: group_lad<T, ADJ, 1>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
-          case 2: group_lad<T, ADJ, 2>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
-          case 3: group_lad<T, ADJ, 3>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
-          default: group_lad<T, ADJ, 4>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          // exponent sets {1, 4} and {1, 2, 4, 16} have their own instances, others the full one
+          case 1: group_lad<T, ADJ, 1, 0x3>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          case 2:
+            if (grp.lmask == 0x5) group_lad<T, ADJ, 2, 0x5>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            else group_lad<T, ADJ, 2, 0x7>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            break;
+          case 3: group_lad<T, ADJ, 3, 0xf>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          default:
+            if (grp.lmask == 0x17) group_lad<T, ADJ, 4, 0x17>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            else group_lad<T, ADJ, 4, 0x1f>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            break;
         }
         wave_sync();
         continue;

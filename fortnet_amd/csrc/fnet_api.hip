@@ -83,7 +83,7 @@ void ladder_form(AngGroup& a) {
     a.smem[s] = -1;
     a.lpdiag[s] = 0.0;
   }
-  int nl = 0, mx = 1;
+  int nl = 0, mx = 1, mask = 0;
   for (int q = 0; q < a.nmem; ++q) {
     int m = -1;
     for (int k = 0; k <= 4; ++k)
@@ -100,6 +100,7 @@ void ladder_form(AngGroup& a) {
     if (a.smem[5 * i + m] >= 0) return;
     a.smem[5 * i + m] = q;
     mx = std::max(mx, m);
+    mask |= 1 << m;
   }
   if (nl == 0) a.lamv[0] = 0.0;
   if (nl < 2) a.lamv[1] = a.lamv[0];
@@ -111,6 +112,7 @@ void ladder_form(AngGroup& a) {
     }
   }
   a.lad = mx;
+  a.lmask = mask;
 }
 
 void invalidate(fnet_batch* b) {

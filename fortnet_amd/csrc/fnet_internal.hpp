@@ -45,6 +45,7 @@ struct AngGroup {
   // Exponent-ladder form (lad > 0): every member has xi = 2^m with m <= lad <= 4 and one of at
   // most two lambda values; slot 5 i + m stands for (lamv[i], 2^m) and names its member (or -1).
   int lad;
+  int lmask;  // bit m set when some member has xi = 2^m
   double lamv[2];
   int smem[kLadSlots];
   double lpdiag[kLadSlots]; // (1 + lamv[i])^(2^m)
