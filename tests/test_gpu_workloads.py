@@ -208,3 +208,53 @@ def test_generic_exponent_path(ctx, name):
     assert (np.abs(Es32[:, 0] - Es64) / nat).max() <= E_TOL
     assert np.sqrt(np.mean((F32 - F0) ** 2)) * EV_BOHR_TO_EV_AA <= F_RMSE_TOL
     b.close()
+
+
+def _clusters(seed, sizes=range(2, 11)):
+    """Non-periodic H/O clusters small enough that every atom sees every other one."""
+    rng = np.random.default_rng(seed)
+    geos = []
+    for n in sizes:
+        while True:
+            xyz = rng.uniform(-2.2, 2.2, size=(n, 3))
+            d = np.linalg.norm(xyz[:, None] - xyz[None], axis=2) + 10 * np.eye(n)
+            if d.min() > 0.9:
+                break
+        z = rng.choice([1, 8], size=n)
+        geos.append(fa.Geometry(xyz * fa.AA__BOHR, z, (z == 8).astype(np.int32)))
+    return geos
+
+
+@pytest.mark.parametrize("generic", [False, True], ids=["ladder", "generic"])
+def test_small_clusters_match_oracle(ctx, generic):
+    """Neighbour lists of 0-9 entries per element: rows of one or two entries, odd and even row
+    lengths, species lists of different lengths in either order, for G4 and G5 groups; the
+    two-pairs-per-step walk of the exponent-ladder kernels (and, with one xi = 3 member, the
+    per-member walk) must give the oracle's values, energies and forces in FP64."""
+    rc = 6.5 * fa.AA__BOHR
+    geos = _clusters(11)
+    rad = fa.GFunctions([2] * 3, [rc] * 3, eta=W.eta_bohr([0.01, 0.1, 1.0]))
+    g4 = fa.GFunctions([4] * 4, [rc] * 4, eta=W.eta_bohr([0.005] * 4), xi=[1, 1, 4, 4],
+                       lam=[1.0, -1.0, 1.0, -1.0])
+    xi5 = [1, 2, 16, 3 if generic else 8]
+    g5 = fa.GFunctions([5] * 4, [rc] * 4, eta=W.eta_bohr([0.02] * 4), xi=xi5,
+                       lam=[1.0, -1.0, 1.0, -1.0])
+    g = fa.GFunctions.concat([rad, g4, g5]).species_resolved([1, 8])
+    dims = [len(g), 8, 1]
+    ns = fa.Netstat(g, [1, 8], dims, 0, W.random_params(dims, 2, 9))
+    ctx.upload_model(ns)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    ns = W.with_zscore(ns, b.acsf())
+    b.close()
+    ctx.upload_model(ns)
+    G0, E0, F0 = _oracle_all(geos, ns)
+    for launch in (fa.FNET_BATCHED, fa.FNET_PER_STRUCT):
+        b = ctx.batch(geos, fa.FNET_FP64)
+        b.set_mode(fa.FNET_FP64, launch)
+        G = b.acsf()
+        _, Ea = b.energy()
+        F = b.forces()
+        b.close()
+        assert np.abs(G - G0).max() <= 1e-10 * max(1.0, np.abs(G0).max())
+        assert np.abs(Ea - E0).max() <= 1e-10 * max(1.0, np.abs(E0).max())
+        assert np.abs(F - F0).max() <= 1e-9 * max(1.0, np.abs(F0).max())
