@@ -3,7 +3,8 @@
 ``python -m fortnet_amd.build`` compiles every ``csrc/*.hip`` translation unit in parallel and
 links the shared library next to this file, where ``fortnet_amd._lib`` loads it; it also links
 ``libfnet_probe.so`` (``probe/``: a plain copy kernel that bench.py uses to scale the HBM byte
-counters when it runs under a profiler).
+counters when it runs under a profiler) and, with amdflang, the Fortran host layer
+(``fortran/``: the ISO_C_BINDING module, the host types and the ``fnet_fortran_check`` driver).
 """
 import concurrent.futures
 import os
@@ -16,6 +17,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfnet_hip.so")
 PROBE_LIB = os.path.join(HERE, "libfnet_probe.so")  # copy kernel for counter calibration
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FNET_FC", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+FCHECK = os.path.join(FORTRAN, "fnet_fortran_check")  # Fortran host driver (executable)
 ARCH = os.environ.get("FNET_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for every device compilation.
@@ -48,6 +52,32 @@ def _link(objs, lib):
     os.replace(tmp, lib)
 
 
+def build_fortran():
+    """Compiles the Fortran host layer and links the driver against libfnet_hip.so (found next
+    to it through the run path, so the tree can move).  Returns the executable, or None without
+    a Fortran compiler."""
+    if not os.path.exists(FC):
+        return None
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for name in ("fnet_hip_c", "fnet_host", "fnet_fortran_check"):
+        obj = os.path.join(mod, name + ".o")
+        cmd = [FC, "-O2", "-module-dir", mod, "-I", mod, "-c",
+               os.path.join(FORTRAN, name + ".f90"), "-o", obj]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed for %s:\n%s" % (name, r.stdout))
+        objs.append(obj)
+    tmp = FCHECK + ".tmp%d" % os.getpid()
+    cmd = [FC] + objs + ["-L" + HERE, "-lfnet_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran link failed:\n" + r.stdout)
+    os.replace(tmp, FCHECK)
+    return FCHECK
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -60,6 +90,7 @@ def build(verbose=False):
                 print(out)
     _link([o for o, _ in results[:-1]], LIB)
     _link([results[-1][0]], PROBE_LIB)
+    build_fortran()
     return LIB
 
 
