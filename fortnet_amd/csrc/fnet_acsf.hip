@@ -910,10 +910,12 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
         const T qi = (T)ext_q(a.ext, a.next, fn.atomid, i);
         double acc = 0.0;
         if (act) {
-          for (int e = gl; e < n; e += ng) {
+          // a species-resolved function sums over its element's range of the staged entries
+          const int e0 = fn.zc >= 0 ? sl.st[fn.zc] : 0;
+          const int e1 = fn.zc >= 0 ? e0 + sl.nz[fn.zc] : n;
+          for (int e = e0 + gl; e < e1; e += ng) {
             const T r = sl.d[e].w;
-            const int zc = sl.zc[e];
-            if ((fn.zc < 0 || fn.zc == zc) && r <= rc) {
+            if (r <= rc) {
               T fcv;
               if (shared) {
                 fcv = sl.fc[e];
@@ -1016,10 +1018,12 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
       const int B0 = grp.zc2 >= 0 ? sl.st[grp.zc2] : 0;
       const int L2 = grp.zc2 >= 0 ? sl.nz[grp.zc2] : n;
       const T qi = (T)ext_q(a.ext, a.next, grp.atomid, i);
-      // per-entry radial factor E = exp(-eta r^2) fc(r) of this group and E'(r) / r
-      for (int e0 = 0; e0 < n; e0 += 64) {
+      // per-entry radial factor E = exp(-eta r^2) fc(r) of this group and E'(r) / r, over the
+      // range the group's two element lists span
+      const int ea = min(A0, B0), eb = max(A0 + L1, B0 + L2);
+      for (int e0 = ea; e0 < eb; e0 += 64) {
         const int e = e0 + lane;
-        if (e >= n) break;
+        if (e >= eb) break;
         const T r = sl.d[e].w;
         T qj = 1, E = 0, Ep = 0;
         if (r <= rc) {
