@@ -127,10 +127,12 @@ struct AcsfArgs {
 // (forward only) or the FP64 adjoint accumulator (adjoint only), the neighbour index and element
 // code, and the start and length of each element's range.  The slice is kept small: the number of
 // resident waves per CU, and with it the adjoint's speed, follows the LDS it takes (a slice 1.5x
-// larger made the adjoint 1.3x slower on the water boxes).
+// larger made the adjoint 1.3x slower on the water boxes).  With paired groups the adjoint slice
+// also holds pb, the partner group's pa.
 template <typename T, bool ADJ>
-__host__ __device__ size_t slice_bytes(int M) {
+__host__ __device__ size_t slice_bytes(int M, int pairs = 0) {
   size_t b = 2 * sizeof(typename Vec4<T>::type) * M;  // d, pa
+  if (ADJ && pairs) b += sizeof(typename Vec4<T>::type) * M;  // pb
   if (!ADJ) b += 64 * sizeof(double) + sizeof(T) * M;  // radial partial sums, fc
   if (ADJ) b += 3 * sizeof(double) * M;               // gs
   b += 2 * sizeof(int) * M;                           // j, zc
@@ -146,6 +148,14 @@ __host__ __device__ inline int slice_entries(int max_nb) {
 // LDS-typed pointer: its index arithmetic stays 32-bit (a generic pointer's is 64-bit)
 typedef __attribute__((address_space(3))) double lds_f64;
 
+// The model's tables and the dE/dG rows do not change during a launch and are read through the
+// constant address space: wave-uniform reads of them then become scalar loads, which the compiler
+// does not emit for generic pointers in a kernel that also writes global memory (as vector loads,
+// the chains slot -> column -> dE/dG kept the Si adjoints waiting on memory ~60% of the time).
+typedef const __attribute__((address_space(4))) AngGroup CGroup;
+typedef const __attribute__((address_space(4))) RadialFn CRadial;
+typedef const __attribute__((address_space(4))) double cdouble;
+
 __device__ __forceinline__ void lds_add(lds_f64* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
@@ -155,6 +165,7 @@ struct Slice {
   using V = typename Vec4<T>::type;
   V* d;
   V* pa;
+  V* pb;  // adjoint with paired groups: the partner group's pa
   T* fc;  // forward only
   lds_f64* gs;  // FP64: ds_add_f64 measured faster than ds_add_f32 here, and exact to more bits
   int *j, *zc;
@@ -163,13 +174,18 @@ struct Slice {
 };
 
 template <typename T, bool ADJ>
-__device__ Slice<T> make_slice(unsigned char* p, int M) {
+__device__ Slice<T> make_slice(unsigned char* p, int M, int pairs) {
   using V = typename Vec4<T>::type;
   Slice<T> s;
   s.d = (V*)p;
   p += sizeof(V) * M;
   s.pa = (V*)p;
   p += sizeof(V) * M;
+  s.pb = nullptr;
+  if (ADJ && pairs) {
+    s.pb = (V*)p;
+    p += sizeof(V) * M;
+  }
   s.red = nullptr;
   s.fc = nullptr;
   if (!ADJ) {
@@ -725,8 +741,8 @@ __device__ __forceinline__ void pairs_adj_lad(const Slice<T>& sl, int lane, int 
 // One ladder group of one centre: values (forward) or the per-entry adjoint accumulation.  MASK:
 // the exponent classes m (xi = 2^m) some member of the group has; the others are not evaluated.
 template <typename T, bool ADJ, int MX, int MASK>
-__device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, long long i, int A0,
-                          int L1, int B0, int L2, T rc, T eta, T pi, T pirc, const double* cG,
+__device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i, int A0,
+                          int L1, int B0, int L2, T rc, T eta, T pi, T pirc, cdouble* cG,
                           double* G, int nG) {
   using V = typename Vec4<T>::type;
   const bool g4 = grp.type == 4;
@@ -812,17 +828,150 @@ __device__ void group_lad(const AngGroup& grp, const Slice<T>& sl, int lane, lon
   }
 }
 
+// Adjoint of two G5 ladder groups over the same lists with the same cutoff, lambdas and exponent
+// slots but another eta, in one walk.  Per pair, the cosine, the ladder powers, the loads of the
+// shared entry data and the gradient vectors with their accumulator updates are shared; each
+// group adds its own S0, S1 and radial factors (pa: the first group's E and E' / r, pb: the
+// partner's), so the coefficients of adj_coef become sums over the two groups:
+//   t = S1a Ea_j Ea_k + S1b Eb_j Eb_k,  Q = t / (r_j r_k),
+//   Pj = S0a Ea_k Ea'_j / r_j + S0b Eb_k Eb'_j / r_j - c t / r_j^2.
+template <typename T, int MX, int MASK, bool SAME>
+__device__ __forceinline__ void pairs_adj_lad2(const Slice<T>& sl, int lane, int A0, int LA,
+                                               int B0, int LB, T lam0, T lam1,
+                                               const T (&c0)[2][2][MX + 1],
+                                               const T (&c1)[2][2][MX + 1],
+                                               const T (&c1const)[2]) {
+  using V = typename Vec4<T>::type;
+  PairWalk2<SAME> it;
+  it.init(lane, A0, LA, B0, LB);
+  int ej, ek1, ek2;
+  bool v1, v2;
+  auto pair = [&](const V& dj, const V& pj, const V& qj, int ek, bool v, T& gx, T& gy, T& gz) {
+    const V dk = sl.d[ek], pk = sl.pa[ek], qk = sl.pb[ek];
+    const T rr = pj.x * pk.x;
+    const T c = fmin(fmax((dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * rr, (T)-1), (T)1);
+    T S0a = 0, S1a = c1const[0], S0b = 0, S1b = c1const[1];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      T p = (T)1 + (l ? lam1 : lam0) * c;
+      T pm = (T)1;
+      if (MASK & 1) {
+        S0a += c0[0][l][0] * p;
+        S0b += c0[1][l][0] * p;
+      }
+#pragma unroll
+      for (int m = 1; m <= MX; ++m) {
+        pm = pm * p;
+        p = p * p;
+        if (MASK & (1 << m)) {
+          S0a += c0[0][l][m] * p;
+          S1a += c1[0][l][m] * pm;
+          S0b += c0[1][l][m] * p;
+          S1b += c1[1][l][m] * pm;
+        }
+      }
+    }
+    // a pair that does not exist adds nothing
+    S0a = v ? S0a : (T)0;
+    S1a = v ? S1a : (T)0;
+    S0b = v ? S0b : (T)0;
+    S1b = v ? S1b : (T)0;
+    const T t = S1a * (pj.y * pk.y) + S1b * (qj.y * qk.y);
+    const T Q = t * rr;
+    const T ct = c * t;
+    const T Pj = S0a * pk.y * pj.w + S0b * qk.y * qj.w - ct * (pj.x * pj.x);
+    const T Pk = S0a * pj.y * pk.w + S0b * qj.y * qk.w - ct * (pk.x * pk.x);
+    gx += Pj * dj.x + Q * dk.x;
+    gy += Pj * dj.y + Q * dk.y;
+    gz += Pj * dj.z + Q * dk.z;
+    lds_f64* gk = sl.gs + __umul24(ek, 3);
+    lds_add(gk + 0, (double)(Pk * dk.x + Q * dj.x));
+    lds_add(gk + 1, (double)(Pk * dk.y + Q * dj.y));
+    lds_add(gk + 2, (double)(Pk * dk.z + Q * dj.z));
+  };
+  while (it.next(ej, ek1, ek2, v1, v2)) {
+    const V dj = sl.d[ej], pj = sl.pa[ej], qj = sl.pb[ej];
+    T gx = 0, gy = 0, gz = 0;
+    pair(dj, pj, qj, ek1, v1, gx, gy, gz);
+    pair(dj, pj, qj, ek2, v2, gx, gy, gz);
+    lds_f64* gj = sl.gs + __umul24(ej, 3);
+    lds_add(gj + 0, (double)gx);
+    lds_add(gj + 1, (double)gy);
+    lds_add(gj + 2, (double)gz);
+  }
+}
+
+// Adjoint of a G5 ladder group (ga) and its partner (gb) of another eta; sl.pa holds ga's
+// per-entry factors, sl.pb gb's.  The coefficients are those of group_lad, one set per group.
+template <typename T, int MX, int MASK>
+__device__ void group_lad_pair(CGroup& ga, CGroup& gb, const Slice<T>& sl,
+                               int lane, int A0, int L1, int B0, int L2, cdouble* cG) {
+  using V = typename Vec4<T>::type;
+  const bool same = ga.same != 0;
+  const T lam0 = uniform((T)ga.lamv[0]), lam1 = uniform((T)ga.lamv[1]);
+  const T w = same ? (T)2 : (T)1;  // unordered pairs of one list count twice
+  T c0[2][2][MX + 1], c1[2][2][MX + 1], c1const[2], S0d[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    CGroup& gr = h ? gb : ga;
+    T sd = 0;
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) {
+        const int q = gr.smem[5 * l + m];
+        const T v = q >= 0 ? (T)(cG[gr.col[q]] * gr.scale[q]) : (T)0;
+        c0[h][l][m] = uniform(w * v);
+        c1[h][l][m] = uniform(w * v * (T)(1 << m) * (l ? lam1 : lam0));
+        sd += v * (T)gr.lpdiag[5 * l + m];
+      }
+    S0d[h] = uniform(sd);
+    c1const[h] = uniform(c1[h][0][0] + c1[h][1][0]);
+  }
+  if (same) {
+    // j == k terms of both groups (acsf.F90:1580-1585); each lane owns its entries here
+    for (int e = lane; e < L1; e += 64) {
+      const int ej = A0 + e;
+      const V dj = sl.d[ej], pj = sl.pa[ej], qj = sl.pb[ej];
+      const T gsc = (T)2 * (S0d[0] * pj.y * pj.w + S0d[1] * qj.y * qj.w);
+      sl.gs[3 * ej + 0] += gsc * dj.x;
+      sl.gs[3 * ej + 1] += gsc * dj.y;
+      sl.gs[3 * ej + 2] += gsc * dj.z;
+    }
+    pairs_adj_lad2<T, MX, MASK, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const);
+  } else {
+    pairs_adj_lad2<T, MX, MASK, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, c0, c1, c1const);
+  }
+}
+
+// One ladder group, or (adjoint, pg set) a G5 group with its partner.
+template <typename T, bool ADJ, int MX, int MASK>
+__device__ __forceinline__ void lad_one(CGroup& grp, CGroup* pg,
+                                        const Slice<T>& sl, int lane, long long i, int A0, int L1,
+                                        int B0, int L2, T rc, T eta, T pi, T pirc,
+                                        cdouble* cG, double* G, int nG) {
+  if constexpr (ADJ) {
+    if (pg) {
+      group_lad_pair<T, MX, MASK>(grp, *pg, sl, lane, A0, L1, B0, L2, cG);
+      return;
+    }
+  }
+  group_lad<T, ADJ, MX, MASK>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, G, nG);
+}
+
 // LAD: every angular group has the exponent-ladder form (AngGroup::lad > 0); MQ is then unused.
-template <typename T, bool ADJ, int MQ, bool LAD>
+// PAIRS (adjoint with ladder groups only): paired groups are evaluated together (own instance, so
+// that the kernels of models without pairs keep their register budget).
+template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS>
 __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long i = a.a0 + (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (i >= a.a1) return;
   const int M = a.maxnb;
-  Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M) * wave, M);
+  Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M, PAIRS) * wave, M, PAIRS);
 
   const long long b = a.cap ? i * a.cap : (long long)a.rowptr[i];
   const int n = a.cap ? a.cnt[i] : (int)(a.rowptr[i + 1] - b);
@@ -894,7 +1043,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
 
   const int nouts = ADJ ? a.nout : 1;
   for (int t = 0; t < nouts; ++t) {
-    const double* cG = ADJ ? a.cG + (i * a.nout + t) * a.nG : nullptr;
+    cdouble* cG = ADJ ? (cdouble*)a.cG + (i * a.nout + t) * a.nG : nullptr;
 
     // ---- radial functions ----
     if (!ADJ) {
@@ -952,7 +1101,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
         const T fcl = (T)0.5 * (fcos(xl) + (T)1), dfcl = -(T)0.5 * pircl * fsin(xl);
         T s = 0;
         for (int f = 0; f < a.nR; ++f) {
-          const RadialFn& fn = a.radial[f];
+          CRadial& fn = ((CRadial*)a.radial)[f];
           const T cf = (T)cG[fn.col];
           if (cf == (T)0) continue;
           if (!(fn.zc < 0 || fn.zc == zc) || !(r <= (T)fn.rc)) continue;
@@ -991,14 +1140,19 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
 
     // ---- angular groups ----
     for (int g = 0; g < a.nAG; ++g) {
-      const AngGroup& grp = a.groups[g];
+      CGroup& grp = ((CGroup*)a.groups)[g];
       const int nm = grp.nmem;
       T A[MQ];
       if (ADJ) {
         // a group whose dE/dG are all zero adds nothing to the forces
         bool any = false;
         if (LAD) {
+          if (PAIRS && grp.follower) continue;  // evaluated together with its partner
           for (int q = 0; q < nm; ++q) any = any || (cG[grp.col[q]] != 0.0);
+          if (PAIRS && grp.partner >= 0) {
+            CGroup& pg = ((CGroup*)a.groups)[grp.partner];
+            for (int q = 0; q < pg.nmem; ++q) any = any || (cG[pg.col[q]] != 0.0);
+          }
         } else {
 #pragma unroll
           for (int q = 0; q < MQ; ++q) {
@@ -1018,6 +1172,9 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
       const int B0 = grp.zc2 >= 0 ? sl.st[grp.zc2] : 0;
       const int L2 = grp.zc2 >= 0 ? sl.nz[grp.zc2] : n;
       const T qi = (T)ext_q(a.ext, a.next, grp.atomid, i);
+      // adjoint of a group evaluated with its partner: the partner's factors go to sl.pb
+      CGroup* pg = (PAIRS && grp.partner >= 0) ? &((CGroup*)a.groups)[grp.partner] : nullptr;
+      const T eta2 = pg ? (T)pg->eta : (T)0;
       // per-entry radial factor E = exp(-eta r^2) fc(r) of this group and E'(r) / r, over the
       // range the group's two element lists span
       const int ea = min(A0, B0), eb = max(A0 + L1, B0 + L2);
@@ -1025,7 +1182,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
         const int e = e0 + lane;
         if (e >= eb) break;
         const T r = sl.d[e].w;
-        T qj = 1, E = 0, Ep = 0;
+        T qj = 1, E = 0, Ep = 0, E2 = 0, Ep2 = 0;
         if (r <= rc) {
           T fc, dfc;
           if (shared) {
@@ -1046,6 +1203,11 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
           const T ex = fexp(-eta * r * r);
           E = ex * fc;
           if (ADJ) Ep = ex * (dfc - (T)2 * eta * r * fc);
+          if (PAIRS && pg) {
+            const T ex2 = fexp(-eta2 * r * r);
+            E2 = ex2 * fc;
+            Ep2 = ex2 * (dfc - (T)2 * eta2 * r * fc);
+          }
         }
         V pa;
         pa.x = frcp(r);
@@ -1053,21 +1215,29 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
         pa.z = qj;
         pa.w = Ep * pa.x;
         sl.pa[e] = pa;
+        if (PAIRS && pg) {
+          V pb;
+          pb.x = pa.x;
+          pb.y = E2;
+          pb.z = qj;
+          pb.w = Ep2 * pa.x;
+          sl.pb[e] = pb;
+        }
       }
       wave_sync();
 
       if constexpr (LAD) {
         switch (grp.lad) {
           // exponent sets {1, 4} and {1, 2, 4, 16} have their own instances, others the full one
-          case 1: group_lad<T, ADJ, 1, 0x3>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          case 1: lad_one<T, ADJ, 1, 0x3>(grp, pg, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
           case 2:
-            if (grp.lmask == 0x5) group_lad<T, ADJ, 2, 0x5>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
-            else group_lad<T, ADJ, 2, 0x7>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            if (grp.lmask == 0x5) lad_one<T, ADJ, 2, 0x5>(grp, pg, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            else lad_one<T, ADJ, 2, 0x7>(grp, pg, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
             break;
-          case 3: group_lad<T, ADJ, 3, 0xf>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
+          case 3: lad_one<T, ADJ, 3, 0xf>(grp, pg, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG); break;
           default:
-            if (grp.lmask == 0x17) group_lad<T, ADJ, 4, 0x17>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
-            else group_lad<T, ADJ, 4, 0x1f>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            if (grp.lmask == 0x17) lad_one<T, ADJ, 4, 0x17>(grp, pg, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
+            else lad_one<T, ADJ, 4, 0x1f>(grp, pg, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, a.G, a.nG);
             break;
         }
         wave_sync();
@@ -1171,9 +1341,9 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   }
 }
 
-template <typename T, bool ADJ, int MQ, bool LAD>
+template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS>
 __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
-  acsf_wave<T, ADJ, MQ, LAD>(a);
+  acsf_wave<T, ADJ, MQ, LAD, PAIRS>(a);
 }
 
 // The FP32 exponent-ladder kernels fit in 96 registers without spilling, which leaves room for 5
@@ -1181,7 +1351,13 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
 template <bool ADJ>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5)))
     k_acsf_lad32(AcsfArgs a) {
-  acsf_wave<float, ADJ, 1, true>(a);
+  acsf_wave<float, ADJ, 1, true, false>(a);
+}
+
+// FP32 ladder adjoint with paired groups: 102 registers without spilling at 4 waves per SIMD
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+    k_acsf_lad32_pairs(AcsfArgs a) {
+  acsf_wave<float, true, 1, true, true>(a);
 }
 
 template <typename T, bool ADJ, int MQ, bool LAD>
@@ -1190,7 +1366,9 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
   const int M = slice_entries(b->max_nb);
-  const size_t sb = slice_bytes<T, ADJ>(M);
+  // paired ladder groups share one adjoint walk when the larger slice still fits a block
+  const int pairs = (ADJ && LAD && m.has_pairs && slice_bytes<T, ADJ>(M, 1) <= 64 * 1024) ? 1 : 0;
+  const size_t sb = slice_bytes<T, ADJ>(M, pairs);
   // waves per block (water boxes: adjoint 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 /
   // 2.25 ms with 2 / 4)
   int wpb = ADJ ? 1 : 2;
@@ -1220,10 +1398,19 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
+  if constexpr (ADJ && LAD) {
+    if (pairs) {
+      if constexpr (std::is_same<T, float>::value)
+        k_acsf_lad32_pairs<<<grid, 64 * wpb, sb * wpb, s>>>(a);
+      else
+        k_acsf<T, true, 1, true, true><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+      return hipGetLastError();
+    }
+  }
   if constexpr (LAD && std::is_same<T, float>::value)
     k_acsf_lad32<ADJ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
   else
-    k_acsf<T, ADJ, MQ, LAD><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+    k_acsf<T, ADJ, MQ, LAD, false><<<grid, 64 * wpb, sb * wpb, s>>>(a);
   return hipGetLastError();
 }
 

@@ -640,7 +640,27 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   m.all_lad = 1;
   for (AngGroup& a : grp) {
     ladder_form(a);
+    a.partner = -1;
+    a.follower = 0;
     if (a.lad == 0) m.all_lad = 0;
+  }
+  // G5 ladder groups that differ only in eta share one pass of the adjoint
+  m.has_pairs = 0;
+  for (size_t p = 0; p < grp.size(); ++p) {
+    AngGroup& a = grp[p];
+    if (a.type != 5 || a.lad == 0 || a.partner >= 0 || a.follower) continue;
+    for (size_t q = p + 1; q < grp.size(); ++q) {
+      AngGroup& b = grp[q];
+      if (b.type != 5 || b.lad != a.lad || b.lmask != a.lmask || b.partner >= 0 || b.follower)
+        continue;
+      if (b.zc1 != a.zc1 || b.zc2 != a.zc2 || b.same != a.same || b.atomid != a.atomid ||
+          b.rc != a.rc || b.lamv[0] != a.lamv[0] || b.lamv[1] != a.lamv[1])
+        continue;
+      a.partner = (int)q;
+      b.follower = 1;
+      m.has_pairs = 1;
+      break;
+    }
   }
   m.need_ext = 0;
   for (const RadialFn& f : rad) m.need_ext = std::max(m.need_ext, f.atomid);

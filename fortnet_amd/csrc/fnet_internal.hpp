@@ -49,6 +49,11 @@ struct AngGroup {
   double lamv[2];
   int smem[kLadSlots];
   double lpdiag[kLadSlots]; // (1 + lamv[i])^(2^m)
+  // Adjoint pass: a G5 ladder group and another one over the same lists with the same ladder
+  // and lambdas but another eta are evaluated together (partner >= 0 on the first one,
+  // follower = 1 on the second).
+  int partner;
+  int follower;
 };
 
 // Network shape of every element network.
@@ -76,6 +81,7 @@ struct DevModel {
   int nG = 0, nR = 0, nAG = 0, nZ = 0;
   int maxmem = 1;               // largest angular group
   int all_lad = 0;              // every angular group has the exponent-ladder form
+  int has_pairs = 0;            // some angular groups are evaluated in pairs by the adjoint
   int need_ext = 0;             // external feature columns the model reads
   int zlist[kMaxZ];
   int nspecies = 0, nout = 0, nin = 0, next_in = 0;
