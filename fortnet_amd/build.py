@@ -20,6 +20,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("FNET_FC", "/opt/rocm/bin/amdflang")
 FORTRAN = os.path.join(HERE, "fortran")
 FCHECK = os.path.join(FORTRAN, "fnet_fortran_check")  # Fortran host driver (executable)
+# HDF5 I/O layer: plain C over the HDF5 C and high-level libraries of the image (HDF5 1.10)
+CC = os.environ.get("FNET_CC", "gcc")
+HDF5_PREFIX = os.environ.get("FNET_HDF5_PREFIX", "/opt/conda")
+H5_LIB = os.path.join(HERE, "libfnet_h5.so")
 ARCH = os.environ.get("FNET_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for every device compilation.
@@ -78,6 +82,26 @@ def build_fortran():
     return FCHECK
 
 
+def build_h5():
+    """Builds libfnet_h5.so (csrc/fnet_h5.c) against the HDF5 libraries under HDF5_PREFIX, found
+    at run time through the run path.  Returns the library, or None without HDF5."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5_hl.h")) and
+            os.path.exists(os.path.join(libdir, "libhdf5_hl.so"))):
+        return None
+    tmp = H5_LIB + ".tmp%d" % os.getpid()
+    cmd = [CC, "-O2", "-std=c99", "-Wall", "-shared", "-fPIC",
+           "-I" + os.path.join(HERE, "..", "include"), "-I" + inc,
+           os.path.join(CSRC, "fnet_h5.c"), "-L" + libdir, "-lhdf5_hl", "-lhdf5",
+           "-Wl,-rpath," + libdir, "-o", tmp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("HDF5 layer build failed:\n" + r.stdout)
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -91,6 +115,7 @@ def build(verbose=False):
     _link([o for o, _ in results[:-1]], LIB)
     _link([results[-1][0]], PROBE_LIB)
     build_fortran()
+    build_h5()
     return LIB
 
 
