@@ -630,8 +630,8 @@ __device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int l
 // P^(2^m) for every slot and, as the running product, P^(2^m - 1) for the derivative: the same
 // straight-line code for every slot, with no per-member branch on the exponent.
 
-// Forward: off-diagonal pairs, two per step (PairWalk2), FP64 partial sums per slot; the two
-// pairs' terms are added in T before the conversion.
+// Forward: off-diagonal pairs, two per step (PairWalk2), FP64 sums per slot; the two pairs'
+// terms are added in T before the conversion.
 template <typename T, int MX, int MASK, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, const JkConst<T>& K,
@@ -659,6 +659,25 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
     }
     return in ? base : (T)0;
   };
+  // G4: the slot sums go through T partial sums that are added to the FP64 ones every 16 steps
+  // (water boxes: forward 1.13 -> 1.09 ms; for the cheaper G5 pairs the extra registers cost more
+  // than the conversions saved, so they add to FP64 every step)
+  constexpr int FLUSH = G4 ? 16 : 1;
+  T part[2][MX + 1];
+#pragma unroll
+  for (int l = 0; l < 2; ++l)
+#pragma unroll
+    for (int m = 0; m <= MX; ++m) part[l][m] = (T)0;
+  auto flush = [&]() {
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) {
+        if (MASK & (1 << m)) acc[l][m] += (double)part[l][m];
+        part[l][m] = (T)0;
+      }
+  };
+  int np = 0;
   while (it.next(ej, ek1, ek2, v1, v2)) {
     const V dj = sl.d[ej], pj = sl.pa[ej];
     const V dk1 = sl.d[ek1], pk1 = sl.pa[ek1];
@@ -670,15 +689,20 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
     for (int l = 0; l < 2; ++l) {
       const T lam = l ? lam1 : lam0;
       T p1 = (T)1 + lam * c1, p2 = (T)1 + lam * c2;
-      if (MASK & 1) acc[l][0] += (double)(b1 * p1 + b2 * p2);
+      if (MASK & 1) part[l][0] += b1 * p1 + b2 * p2;
 #pragma unroll
       for (int m = 1; m <= MX; ++m) {
         p1 = p1 * p1;
         p2 = p2 * p2;
-        if (MASK & (1 << m)) acc[l][m] += (double)(b1 * p1 + b2 * p2);
+        if (MASK & (1 << m)) part[l][m] += b1 * p1 + b2 * p2;
       }
     }
+    if (++np == FLUSH) {
+      np = 0;
+      flush();
+    }
   }
+  flush();
 }
 
 // Adjoint: off-diagonal pairs, two per step (PairWalk2) sharing j, whose gradient is summed in
