@@ -217,6 +217,13 @@ struct fnet_batch {
 
 namespace fnet {
 
+// Distance stored with a neighbour entry (V.w).  FP64 rows: the square root of the FP64 squared
+// distance of the membership test.  FP32 rows (the FP32 path, held to its accuracy gates): the
+// FP32 square root of the rounded squared distance, which spares the search loops the FP64 square
+// root's refinement steps.  The membership test itself is FP64 in both.
+__device__ __forceinline__ void entry_dist(double4& v, double d2) { v.w = sqrt(d2); }
+__device__ __forceinline__ void entry_dist(float4& v, double d2) { v.w = sqrtf((float)d2); }
+
 // launchers (fnet_neigh.hip)
 hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s);
 hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
