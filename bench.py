@@ -104,7 +104,7 @@ def measured_traffic(stage):
 def cpu_baseline(geos, ns, budget_s=20.0):
     """The FP64 C oracle in the reference's loop shape (per-atom/per-function neighbour rebuild,
     dense dG/dr, dense force contraction) over the workload's structures: on one core for
-    ``budget_s`` seconds, then on all the cores the box grants (at most 16) with structures
+    ``budget_s`` seconds, then on all the cores the process may use (at most 16) with structures
     handed out to threads, as the reference splits them over MPI ranks (parallel.F90:23-56),
     for half that time.  The oracle's C calls release the interpreter lock."""
     import concurrent.futures
@@ -234,8 +234,8 @@ def main():
                     help="end with a plain 512 MiB device copy (x3) to scale HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(structures shard with no exchange for inference; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the device path has no multi-GPU launch "
+              "(structures shard with no exchange for inference)", file=sys.stderr)
         sys.exit(2)
 
     geos, ns, desc = W.config(args.workload, args.scale)

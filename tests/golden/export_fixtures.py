@@ -4,8 +4,8 @@ This script is run once, in the development container, with an interpreter that
 has ``h5py`` (``/opt/conda/bin/python3.9 tests/golden/export_fixtures.py``).  It
 reads data only -- the datasets, netstat and fnetout HDF5 files held under
 ``test/prog/fortnet`` of the reference -- and writes plain arrays, so nothing of
-the reference is executed or copied as source.  The GPU box never reads the
-reference; it only sees the ``.npz`` files written here.
+the reference is executed or copied as source.  The GPU tests never read the
+reference; they only see the ``.npz`` files written here.
 
 Layout conventions (Fortran writes column-major arrays, h5py shows them
 transposed):
