@@ -283,11 +283,14 @@ struct PairWalk {
 // valid entry index.
 template <bool SAME>
 struct PairWalk2 {
-  int A0, B0, LA, LB;
-  int j, e, k1, u, uend;
-  int rs, qs, kstep, kwrap, dfull, dhalf;
+  // absolute entry indices J (list A) and K1 (list B) with their ends; Jhalf: entries below it
+  // also take the half offset d = LA / 2 of an even same-list walk
+  int J, Jend, Jhalf, K1, Kend, LA, LB;
+  int d1, u, uend;
+  int rs, dstep, kstep, kwrap, dfull;
 
   __device__ __forceinline__ void init(int lane, int a0, int l1, int b0, int l2) {
+    int A0, B0;
     if (!SAME && l2 > l1) {
       A0 = b0;
       LA = l2;
@@ -299,48 +302,55 @@ struct PairWalk2 {
       B0 = b0;
       LB = l2;
     }
-    j = e = k1 = rs = qs = kstep = kwrap = dfull = dhalf = 0;
+    J = Jend = Jhalf = K1 = Kend = d1 = rs = dstep = kstep = kwrap = dfull = 0;
     u = lane;
     uend = 0;
     if (LA <= 0 || LB <= 0) return;
-    int dtot;
-    if (SAME) {  // unordered pairs: d = 1 .. (LA - 1) / 2, and d = LA / 2 for j < LA / 2
+    int dtot, dhalf = 0;
+    if (SAME) {
       dfull = (LA - 1) / 2;
       dhalf = (LA & 1) == 0 ? LA / 2 : 0;
       dtot = dfull + (dhalf ? 1 : 0);
-    } else {  // every (j, k): d = 0 .. LB - 1
+    } else {
       dfull = LB - 1;
       dtot = LB;
     }
     uend = LA * ((dtot + 1) / 2);
-    j = lane % LA;
-    e = lane / LA;
-    k1 = (j + (SAME ? 1 : 0) + 2 * e) % LB;
-    qs = 64 / LA;
+    const int j = lane % LA, e = lane / LA;
+    d1 = (SAME ? 1 : 0) + 2 * e;
+    J = A0 + j;
+    Jend = A0 + LA;
+    Jhalf = A0 + dhalf;
+    K1 = B0 + (j + d1) % LB;
+    Kend = B0 + LB;
+    const int qs = 64 / LA;
     rs = 64 % LA;
+    dstep = 2 * qs;
     kstep = (rs + 2 * qs) % LB;
     kwrap = ((2 - LA) % LB + LB) % LB;
   }
 
   __device__ __forceinline__ bool next(int& ej, int& ek1, int& ek2, bool& v1, bool& v2) {
     if (u >= uend) return false;
-    const int d1 = (SAME ? 1 : 0) + 2 * e;
-    v1 = d1 <= dfull || (SAME && d1 == dhalf && j < dhalf);
-    v2 = d1 + 1 <= dfull || (SAME && d1 + 1 == dhalf && j < dhalf);
-    const int k2 = k1 + 1 >= LB ? k1 + 1 - LB : k1 + 1;
-    ej = A0 + j;
-    ek1 = B0 + k1;
-    ek2 = B0 + k2;
+    // the pair (j, d) exists for d <= dlim: d <= dfull, or d = LA / 2 for j < LA / 2
+    const int dlim = SAME ? dfull + (J < Jhalf ? 1 : 0) : dfull;
+    v1 = d1 <= dlim;
+    v2 = d1 < dlim;
+    int K2 = K1 + 1;
+    K2 = K2 >= Kend ? K2 - LB : K2;
+    ej = J;
+    ek1 = K1;
+    ek2 = K2;
     u += 64;
-    j += rs;
-    e += qs;
-    k1 += kstep;
-    k1 = (k1 >= LB) ? k1 - LB : k1;
-    if (j >= LA) {
-      j -= LA;
-      e += 1;
-      k1 += kwrap;
-      k1 = (k1 >= LB) ? k1 - LB : k1;
+    J += rs;
+    d1 += dstep;
+    K1 += kstep;
+    K1 = K1 >= Kend ? K1 - LB : K1;
+    if (J >= Jend) {
+      J -= LA;
+      d1 += 2;
+      K1 += kwrap;
+      K1 = K1 >= Kend ? K1 - LB : K1;
     }
     return true;
   }
