@@ -261,7 +261,7 @@ def read_dataset(path):
         if not f.exists(root):
             raise FnetError("'%s' has no fnetdata/dataset group" % path)
         ndp = int(f.attr(root, "ndatapoints"))
-        next = int(f.attr(root, "nextfeatures", 0))
+        n_ext = int(f.attr(root, "nextfeatures", 0))
         withstruct = int(f.attr(root, "withstructures", 1))
         tr = root + "/training"
         nglob = int(f.attr(tr, "nglobaltargets", 0))
@@ -290,7 +290,7 @@ def read_dataset(path):
             coords.append(xyz)
             gsp.append(f.dataset(geo + "/localattoglobalsp", np.int32).reshape(-1))
             atnum.append(f.dataset(geo + "/localattoatnum", np.int32).reshape(-1))
-            ext.append(f.dataset(dp + "/extfeatures", np.float64).reshape(n, next) if next > 0
+            ext.append(f.dataset(dp + "/extfeatures", np.float64).reshape(n, n_ext) if n_ext > 0
                        else np.zeros((n, 0)))
             aw.append(f.dataset(dp + "/atomicweights", np.float64).reshape(n)
                       if f.exists(dp + "/atomicweights") else np.ones(n))
@@ -304,7 +304,7 @@ def read_dataset(path):
         basis=np.stack(basis), periodic=np.array(periodic, np.int32),
         fractional=np.array(frac, np.int32), coords=np.concatenate(coords),
         globalsp=np.concatenate(gsp), atnum=np.concatenate(atnum),
-        atomicnumbers=atnums, nextfeatures=next, ext=np.concatenate(ext),
+        atomicnumbers=atnums, nextfeatures=n_ext, ext=np.concatenate(ext),
         atomicweights=np.concatenate(aw), nglobaltargets=nglob, natomictargets=natomic,
         globaltargets=np.stack(gt) if nglob > 0 else np.zeros((ndp, 0)),
         atomictargets=np.concatenate(at), weights=np.array(weights, np.int32),
