@@ -634,6 +634,9 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
     }
   }
   m.nR = (int)rad.size();
+  m.radial_resolved = 0;
+  for (const RadialFn& f : rad)
+    if (f.zc >= 0) m.radial_resolved = 1;
   m.nAG = (int)grp.size();
   m.maxmem = 1;
   for (const AngGroup& a : grp) m.maxmem = std::max(m.maxmem, a.nmem);

@@ -82,6 +82,7 @@ struct DevModel {
   int maxmem = 1;               // largest angular group
   int all_lad = 0;              // every angular group has the exponent-ladder form
   int has_pairs = 0;            // some angular groups are evaluated in pairs by the adjoint
+  int radial_resolved = 0;      // some radial function is restricted to one neighbour element
   int need_ext = 0;             // external feature columns the model reads
   int zlist[kMaxZ];
   int nspecies = 0, nout = 0, nin = 0, next_in = 0;
