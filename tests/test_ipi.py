@@ -20,7 +20,8 @@ from fortnet_amd import workloads as W
 
 
 class Server(threading.Thread):
-    """Scripted i-PI server: sends each geometry, collects what the driver returns."""
+    """Scripted i-PI server: sends each geometry, collects what the driver returns.  ``path`` is a
+    UNIX socket path, or None for an internet socket on 127.0.0.1 (port in ``self.address``)."""
 
     def __init__(self, path, frames, natom_sent=None, bad_header=None):
         super().__init__(daemon=True)
@@ -31,8 +32,14 @@ class Server(threading.Thread):
         self.results = []
         self.replies = []
         self.error = None
-        self.lsock = socket.socket(socket.AF_UNIX, socket.SOCK_STREAM)
-        self.lsock.bind(path)
+        if path is None:
+            self.lsock = socket.socket(socket.AF_INET, socket.SOCK_STREAM)
+            self.lsock.bind(("127.0.0.1", 0))
+            self.address = ("127.0.0.1", self.lsock.getsockname()[1])
+        else:
+            self.lsock = socket.socket(socket.AF_UNIX, socket.SOCK_STREAM)
+            self.lsock.bind(path)
+            self.address = path
         self.lsock.listen(1)
 
     def _recv(self, c, n):
@@ -118,6 +125,20 @@ def test_protocol_round_trip(tmp_path):
         assert e == e0 and np.array_equal(f, f0)
         assert np.allclose(vir, abs(np.linalg.det(h)))  # the reference's virial: ones x volume
         assert nx == 0
+
+
+def test_internet_socket(tmp_path):
+    """The reference's internet-socket mode (host and port, ipisocket.F90:55-84) on 127.0.0.1."""
+    frames = _frames(3, 2, seed=4)
+    srv = Server(None, frames)
+    srv.start()
+    drv = ipi.IpiDriver(srv.address, [8, 1, 1], evaluate=_harmonic, timeout=30)
+    assert drv.run(5) == 2
+    drv.close()
+    srv.join(30)
+    assert srv.error is None, srv.error
+    for (coords, h), (e, f, vir, nx) in zip(frames, srv.results):
+        assert e == float((coords ** 2).sum()) and np.array_equal(f, -2.0 * coords)
 
 
 def test_step_limit_and_wrong_atom_count(tmp_path):
