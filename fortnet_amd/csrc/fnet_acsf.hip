@@ -1191,7 +1191,20 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
         const T fcl = (T)0.5 * (fcos(xl) + (T)1), dfcl = -(T)0.5 * pircl * fsin(xl);
         T s = 0;
         for (int f = 0; f < a.nR; ++f) {
-          CRadial& fn = ((CRadial*)a.radial)[f];
+          // the whole record, held in scalar registers from here on (the empty asm takes every
+          // field as an operand): its loads issue together instead of one after each branch
+          CRadial& fr = ((CRadial*)a.radial)[f];
+          RadialFn fn;
+          fn.col = fr.col;
+          fn.type = fr.type;
+          fn.zc = fr.zc;
+          fn.atomid = fr.atomid;
+          fn.rc = fr.rc;
+          fn.eta = fr.eta;
+          fn.rs = fr.rs;
+          fn.kappa = fr.kappa;
+          __asm__ volatile("" ::"s"(fn.col), "s"(fn.type), "s"(fn.zc), "s"(fn.atomid), "s"(fn.rc),
+                           "s"(fn.eta), "s"(fn.rs), "s"(fn.kappa));
           const T cf = (T)cG[fn.col];
           if (cf == (T)0) continue;
           if (!(fn.zc < 0 || fn.zc == zc) || !(r <= (T)fn.rc)) continue;
