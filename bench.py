@@ -1,19 +1,24 @@
 """Benchmark of the BPNN energy+force path on one MI355X.
 
-Workload (default): 500 periodic boxes of 64 rigid H2O molecules (192 atoms, 12.42 A cube,
-96 000 atoms), species-resolved ACSF (2 x 10 G2 + 3 x 4 G4 = 32 features, Rc = 6.5 A), one
-32-25-25-1 tanh network per element (H, O), seeded random weights, z-score over the batch.
+Workload (default, the batching configuration): 8192 periodic Si cells of 8-512 atoms
+(log-uniform, 997 112 atoms, density 2.33 +- 5 % g/cm^3, minimum distance 2 A, seeded), 24 ACSF
+(8 G2 + 16 G5, Rc = 6 A), one 24-64-64-1 tanh network, seeded random weights, z-score over the
+batch.  ``--workload`` selects the others (water boxes, Si64, the 131 072-atom Si cell, ternary
+cells).
 
 One step = one full energy+force evaluation of the whole batch with the inputs resident in
-HBM: periodic neighbour list, ACSF, element networks with the input gradient, adjoint forces and
-the FP64 per-structure energy reduction (fnet_run).  ``value`` is atoms per second of the
-batched FP32 path; the plain path (FP64, one structure per launch) is timed on the same batch
-and reported beside it, as are the FP32-vs-FP64 energy and force errors.
+HBM: periodic neighbour list (with the mirror entries of the force gather), ACSF, element networks
+with the input gradient, adjoint forces and the FP64 per-structure energy reduction (fnet_run).
+Each precision mode (FP32; BF16 hidden layers with split operands; BF16 single pass) is held
+against the FP64 device path (itself pinned to the oracle): energy error per atom and force RMSE.
+``value`` is atoms per second of the fastest mode within the gates (1e-5 eV/atom, 1e-4 eV/A),
+eagerly launched or replayed as a HIP graph, whichever is faster; the plain path (FP64, one
+structure per launch) is timed on the same batch and reported beside it.
 
 ``roofline`` is for the dominant kernel (largest stage time measured with HIP events on the
 stream the kernels run on); ``cpu_baseline`` times the FP64 C oracle in the reference's loop
 shape (neighbour list rebuilt per atom and function, dense dG/dr, dense force contraction) on a
-few structures of the same workload, on one host core.
+bounded sample of the same workload.
 """
 import argparse
 import json
@@ -31,6 +36,7 @@ from fortnet_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
 def algorithmic_bytes(stage, N, P, nG):
@@ -84,11 +90,11 @@ def acsf_flops(batch, geos, ns):
     return fl
 
 
-def measured_traffic(stage):
-    """HBM bytes per launch of a stage on the default workload, as measured with the FETCH_SIZE
-    and WRITE_SIZE counters (scaled by a plain copy of known size) and recorded in
-    profiles/hbm_traffic_water.md; None when the stage has no measurement there."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_water.md")
+def measured_traffic(workload, stage):
+    """HBM bytes per launch of a stage's dominant kernel on a workload, as measured with the
+    FETCH_SIZE and WRITE_SIZE counters (scaled by a plain copy of known size) and recorded in
+    profiles/hbm_traffic_<workload>.md; None when there is no measurement."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % workload)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -99,6 +105,23 @@ def measured_traffic(stage):
             except ValueError:
                 return None
     return None
+
+
+def cpu_model():
+    """The host CPU's model name and the cores this process may use."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return name, cores
 
 
 def cpu_baseline(geos, ns, budget_s=20.0):
@@ -159,7 +182,9 @@ def cpu_baseline(geos, ns, budget_s=20.0):
         for f in [ex.submit(worker) for _ in range(threads)]:
             f.result()
     dtn = time.perf_counter() - t0
+    model, avail = cpu_model()
     return {"value": state["atoms"] / dtn, "unit": "atoms/s", "cores": threads, "kind": "port",
+            "cpu": "%s (%d cores available to the process)" % (model, avail),
             "sample": "%d of the workload's structures (%d atoms) on %d threads, %.1f s, FP64 C "
                       "oracle in the reference loop shape (per-atom/per-function neighbour "
                       "rebuild, dense dG/dr, dense force contraction)"
@@ -167,52 +192,86 @@ def cpu_baseline(geos, ns, budget_s=20.0):
             "single_core": single}
 
 
-def serving_bench(rates, duration_s, max_requests, serial_rates, serial_requests):
+def serving_bench(rates, duration_s, max_requests, full):
     """Poisson request streams of ternary Si/C/O cells (16-4096 atoms, log-uniform, 48 features,
-    48-64-64-1 networks) through the continuous-batching loop and the one-at-a-time loop."""
+    48-64-64-1 networks), one fresh seeded cell per request (seed 4000 + request number),
+    through the continuous-batching loop and, at every rate, the one-at-a-time loop on the
+    same arrivals.  ``full``: 20 000 requests per rate."""
     from fortnet_amd import serve
-    geos, ns, desc = W.config("ternary", 0.5)
+    geos, ns, desc = W.config("ternary", 1.0)
     ctx = fa.Context(0)
     ctx.upload_model(ns)
     b = ctx.batch(geos, fa.FNET_FP64)
     ns = W.with_zscore(ns, b.acsf())
     b.close()
     ctx.close()
+    n_max = max(int(min(max_requests, max(200, r * duration_s))) for r in rates)
+    t0 = time.perf_counter()
+    cells = [W.ternary_cell(4000 + k) for k in range(n_max)]
+    gen_s = time.perf_counter() - t0
     srv = serve.BatchingServer(ns, fa.FNET_FP32)
-    srv.run([serve.Request(k, g, 0.0) for k, g in enumerate(geos)])  # warm-up, buffer growth
-    out = {"workload": desc, "precision": "fp32", "mean_atoms_per_request":
-           float(np.mean([g.n for g in geos])), "rates": {}}
+    per_atom = srv.calibrate(cells[:32])
+    srv.run([serve.Request(k, g, 0.0) for k, g in enumerate(cells[:64])])  # warm-up
+    out = {"workload": desc.replace("64 ternary", "fresh seeded ternary") +
+           " (one new cell per request)", "precision": "fp32",
+           "mean_atoms_per_request": float(np.mean([g.n for g in cells])),
+           "device_bytes_per_atom": per_atom, "cells_generated_s": gen_s, "rates": {}}
     for rate in rates:
-        n = int(min(max_requests, max(50, rate * duration_s)))
+        n = int(min(max_requests, max(200, rate * duration_s)))
         arr = serve.poisson_arrivals(rate, n, 4000 + int(rate))
-        reqs = [serve.Request(k, geos[k % len(geos)], float(t)) for k, t in enumerate(arr)]
+        reqs = [serve.Request(k, cells[k], float(t)) for k, t in enumerate(arr)]
         srv.batch_sizes = []
+        srv.budgets = []
         srv.run(reqs)
         entry = {"batched": serve.latency_summary(reqs),
-                 "mean_requests_per_batch": float(np.mean(srv.batch_sizes))}
-        if rate in serial_rates:
-            sreqs = [serve.Request(k, geos[k % len(geos)], float(t))
-                     for k, t in enumerate(arr[:serial_requests])]
-            serve.serial_loop(ns, sreqs)
-            entry["serial"] = serve.latency_summary(sreqs)
+                 "mean_requests_per_batch": float(np.mean(srv.batch_sizes)),
+                 "min_atom_budget": int(min(srv.budgets))}
+        sreqs = [serve.Request(k, cells[k], float(t)) for k, t in enumerate(arr)]
+        serve.serial_loop(ns, sreqs)
+        entry["serial"] = serve.latency_summary(sreqs)
         out["rates"][str(int(rate))] = entry
         print("serving rate %d: %s" % (rate, json.dumps(entry)), file=sys.stderr, flush=True)
     srv.close()
     return out
 
 
-def time_steps(batch, steps, warmup):
+def time_steps(ctx, batch, steps, warmup, graph):
+    """Wall time of ``steps`` back-to-back fnet_run steps (one synchronisation on each side),
+    then the per-stage HIP-event times of ``steps`` more eager steps (or, for graph replays,
+    the device time of each step)."""
+    ctx.set_graph(graph)
     for _ in range(warmup):
         batch.run()
     batch.sync()
-    stage = np.zeros(5)
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.run()
-        stage += batch.stage_times()
     batch.sync()
     dt = time.perf_counter() - t0
-    return dt, stage / max(steps, 1)
+    replayed = batch.graph_replayed()
+    stage = np.zeros(5)
+    dev = 0.0
+    for _ in range(steps):
+        batch.run()
+        dev += ctx.step_time()
+        if not graph:
+            stage += batch.stage_times()
+    ctx.set_graph(False)
+    return dt, stage / max(steps, 1), dev / max(steps, 1), replayed
+
+
+def network_flops(ns, N):
+    """Algorithmic flops of the element networks over N atoms: forward and input gradient
+    (2 N sum d_l d_{l+1} each), and of the hidden-to-hidden products alone."""
+    d = [int(x) for x in ns.dims]
+    per = sum(d[l] * d[l + 1] for l in range(len(d) - 1))
+    hh = sum(d[l] * d[l + 1] for l in range(1, len(d) - 2))
+    return 2 * 2.0 * N * per, 2 * 2.0 * N * hh
+
+
+MODES = {"fp32": fa.FNET_FP32, "bf16": fa.FNET_BF16, "bf16_fast": fa.FNET_BF16_FAST}
+DTYPES = {"fp32": "fp32", "bf16": "bf16x3+fp32", "bf16_fast": "bf16+fp32"}
+E_GATE, F_GATE = 1e-5, 1e-4  # eV/atom, eV/A against the FP64 path
 
 
 def main():
@@ -220,16 +279,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="water",
+    ap.add_argument("--workload", default="si_cells",
                     choices=["water", "si64", "si_cells", "si_bulk", "ternary"])
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--train-steps", type=int, default=3)
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rates", default="500,2000,5000,20000")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="20 000 requests per rate, a serial loop at every rate")
+    ap.add_argument("--serve-rates", default="500,1000,2000,5000,10000,20000")
     ap.add_argument("--serve-duration", type=float, default=1.0)
-    ap.add_argument("--serve-max-requests", type=int, default=2000)
+    ap.add_argument("--serve-max-requests", type=int, default=1500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a plain 512 MiB device copy (x3) to scale HBM byte counters")
     args = ap.parse_args()
@@ -246,21 +309,47 @@ def main():
     ctx.upload_model(ns)
     N = b.N
     nG = len(ns.g)
+    nat = np.array([g.n for g in geos], float)
 
-    # accuracy of the batched FP32 path against the FP64 device path (itself pinned to the oracle)
+    # accuracy of every mode against the FP64 device path (itself pinned to the oracle)
     b.set_mode(fa.FNET_FP64, fa.FNET_BATCHED)
     E64, _ = b.energy()
     F64 = b.forces()
-    b.set_mode(fa.FNET_FP32, fa.FNET_BATCHED)
-    E32, _ = b.energy()
-    F32 = b.forces()
-    nat = np.array([g.n for g in geos], float)
-    e_err = float((np.abs(E32[:, 0] - E64[:, 0]) / nat).max())
-    f_rmse = float(np.sqrt(np.mean((F32 - F64) ** 2)) / fa.BOHR__AA)
-
-    # batched FP32 path (the metric)
+    modes = {}
+    for name, prec in MODES.items():
+        b.set_mode(prec, fa.FNET_BATCHED)
+        E, _ = b.energy()
+        F = b.forces()
+        e_err = float((np.abs(E[:, 0] - E64[:, 0]) / nat).max())
+        f_rmse = float(np.sqrt(np.mean((F - F64) ** 2)) / fa.BOHR__AA)
+        modes[name] = {"max_energy_err_ev_per_atom": e_err, "force_rmse_ev_per_A": f_rmse,
+                       "within_gates": bool(e_err <= E_GATE and f_rmse <= F_GATE)}
     P = b.neighbour_list()
-    dt, stage = time_steps(b, args.steps, args.warmup)
+
+    # timing of every mode, eager and as a HIP graph
+    best = None
+    for name, prec in MODES.items():
+        b.set_mode(prec, fa.FNET_BATCHED)
+        b.run()
+        b.sync()
+        dt, stage, dev, _ = time_steps(ctx, b, args.steps, args.warmup, False)
+        m = modes[name]
+        m["eager"] = {"atoms_per_s": args.steps * N / dt, "ms_per_step": 1e3 * dt / args.steps,
+                      "device_ms_per_step": dev,
+                      "stages_ms": dict(zip(["neighbour", "acsf", "networks", "adjoint",
+                                             "reduce"], [float(x) for x in stage]))}
+        cands = [("eager", dt, stage)]
+        if not args.no_graph:
+            dtg, _, devg, replayed = time_steps(ctx, b, args.steps, args.warmup, True)
+            m["graph"] = {"atoms_per_s": args.steps * N / dtg,
+                          "ms_per_step": 1e3 * dtg / args.steps, "device_ms_per_step": devg,
+                          "replayed": replayed}
+            if replayed:
+                cands.append(("graph", dtg, stage))
+        for how, t, st in cands:
+            if m["within_gates"] and (best is None or t < best[2]):
+                best = (name, how, t, st)
+    name, how, dt, stage = best
     value = args.steps * N / dt
     ms_per_step = 1e3 * dt / args.steps
 
@@ -268,14 +357,32 @@ def main():
     b.set_mode(fa.FNET_FP64, fa.FNET_PER_STRUCT)
     plain, stagep = None, np.zeros(5)
     if args.plain_steps > 0:
-        dtp, stagep = time_steps(b, args.plain_steps, 1)
-        plain = args.plain_steps * N / dtp
+        b.run()
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            b.run()
+            stagep += b.stage_times()
+        b.sync()
+        plain = args.plain_steps * N / (time.perf_counter() - t0)
+        stagep /= args.plain_steps
+
+    # weight gradient (training step without the optimizer) in the headline mode
+    b.set_mode(MODES[name], fa.FNET_BATCHED)
+    rng = np.random.default_rng(0)
+    targets = E64[:, :1] + rng.normal(size=(b.S, 1))
+    b.train_gradients(targets)
+    t0 = time.perf_counter()
+    for _ in range(args.train_steps):
+        b.train_gradients(targets)
+    t_train = (time.perf_counter() - t0) / max(args.train_steps, 1)
+    fl_net, fl_hh = network_flops(ns, N)
 
     names = ["neighbour", "acsf", "networks", "adjoint", "reduce"]
     k = int(np.argmax(stage))
     ab = algorithmic_bytes(names[k], N, P, nG)
     achieved = ab / (stage[k] * 1e-3) / 1e9
-    traffic = measured_traffic(names[k]) if (args.workload, args.scale) == ("water", 1.0) else None
+    traffic = (measured_traffic(args.workload, names[k]) if args.scale == 1.0 else None)
     roofline = {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "algorithmic_bytes": ab, "kernel_ms": float(stage[k])}
@@ -284,31 +391,46 @@ def main():
         tf = fl / (stage[k] * 1e-3) / 1e12
         roofline["valu"] = {"algorithmic_flops": fl, "achieved": tf, "peak": FP32_PEAK_TFLOPS,
                             "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS}
+    net_ms = float(stage[2])
+    mfma = {"kernel": "k_mlp_mfma", "algorithmic_flops": fl_net, "ms": net_ms,
+            "achieved_tflops": fl_net / (net_ms * 1e-3) / 1e12 if net_ms > 0 else None,
+            "peak_fp32_tflops": FP32_PEAK_TFLOPS, "peak_bf16_tflops": BF16_PEAK_TFLOPS,
+            "hidden_to_hidden_flops": fl_hh}
+    if mfma["achieved_tflops"]:
+        mfma["frac_of_fp32_peak"] = mfma["achieved_tflops"] / FP32_PEAK_TFLOPS
+        mfma["frac_of_bf16_peak"] = mfma["achieved_tflops"] / BF16_PEAK_TFLOPS
 
     out = {
         "metric": "atoms/s energy+forces, batched vs plain GPU @1e-5 eV/atom; serve p50/p99 ms",
         "value": value, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[name],
         "data": "synthetic (seeded structures, random-init weights)",
         "config": {"workload": args.workload, "description": desc, "n_structures": len(geos),
-                   "atoms": N, "pairs": P, "n_features": nG,
-                   "precision": "fp32 ACSF/networks/forces, fp64 energy reduction"},
+                   "atoms": N, "pairs": P, "n_features": nG, "precision_mode": name,
+                   "launch": how},
         "plain_fp64_per_structure_atoms_per_s": plain,
         "batched_over_plain": value / plain if plain else None,
-        "fp32_vs_fp64": {"max_energy_err_ev_per_atom": e_err, "force_rmse_ev_per_A": f_rmse},
+        "precision_modes_vs_fp64": modes,
         "stages_ms": dict(zip(names, [float(x) for x in stage])),
         "plain_stages_ms": dict(zip(names, [float(x) for x in stagep])),
+        "train_gradients": {"precision_mode": name, "ms": 1e3 * t_train,
+                            "algorithmic_flops": 1.5 * fl_net,
+                            "note": "forward, backward and weight gradient of every atom's "
+                                    "network, loss, FP64 reductions, gradient download"},
         "roofline": roofline,
+        "mfma": mfma,
     }
     b.close()
     ctx.close()
     if not args.no_serve:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        out["serving"] = serving_bench(rates, args.serve_duration, args.serve_max_requests,
-                                       set(rates[:2]), 300)
+        nmax = 20000 if args.serve_full else args.serve_max_requests
+        out["serving"] = serving_bench(rates, args.serve_duration if not args.serve_full else
+                                       1e9, nmax, args.serve_full)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(geos, ns, args.cpu_budget)
+        cb = cpu_baseline(geos, ns, args.cpu_budget)
+        out["cpu_baseline"] = cb
     if args.copy_calibration:
         import ctypes
         probe = ctypes.CDLL(os.path.join(ROOT, "fortnet_amd", "libfnet_probe.so"))

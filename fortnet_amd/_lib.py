@@ -14,9 +14,12 @@ LIB_PATH = os.path.join(HERE, "libfnet_hip.so")
 FNET_FP64 = 0
 FNET_FP32 = 1
 FNET_FP32_NET64 = 2
+FNET_BF16 = 3        # hidden-to-hidden layers on BF16 MFMA, split hi + lo operands
+FNET_BF16_FAST = 4   # hidden-to-hidden layers on BF16 MFMA, one pass
 FNET_BATCHED = 0
 FNET_PER_STRUCT = 1
 LOSSES = {"mse": 0, "mae": 1, "mape": 2, "rms": 3}  # fnet_loss
+OPTIMIZERS = {"sd": 0, "cg": 1, "lbfgs": 2, "fire": 3}  # fnet_optimizer
 
 EXPORTS = [
     "fnet_ctx_create", "fnet_ctx_destroy", "fnet_last_error", "fnet_params_per_species",
@@ -25,7 +28,8 @@ EXPORTS = [
     "fnet_neighbour_list", "fnet_get_neighbours", "fnet_acsf_forward", "fnet_energy",
     "fnet_forces", "fnet_run", "fnet_train_gradients", "fnet_optimizer_step_sd", "fnet_sync",
     "fnet_stage_times", "fnet_batch_reload", "fnet_query", "fnet_set_loss",
-    "fnet_set_regularization",
+    "fnet_set_regularization", "fnet_set_graph", "fnet_batch_sync", "fnet_step_time",
+    "fnet_batch_step_info", "fnet_optimizer_init", "fnet_optimizer_step", "fnet_device_memory",
 ]
 
 
@@ -91,6 +95,13 @@ def load():
         "fnet_set_regularization": (i32, [vp, dbl, dbl]),
         "fnet_batch_reload": (i32, [vp, ctypes.POINTER(BatchDesc)]),
         "fnet_query": (i32, [vp]),
+        "fnet_set_graph": (i32, [vp, i32]),
+        "fnet_batch_sync": (i32, [vp]),
+        "fnet_step_time": (i32, [vp, ctypes.POINTER(dbl)]),
+        "fnet_batch_step_info": (i32, [vp, ctypes.POINTER(i32)]),
+        "fnet_optimizer_init": (i32, [vp, i32, dbl, dbl, dbl, dbl, i32, i32]),
+        "fnet_optimizer_step": (i32, [vp, i32, vp, ctypes.POINTER(i32)]),
+        "fnet_device_memory": (i32, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

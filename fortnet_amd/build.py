@@ -102,6 +102,22 @@ def build_h5():
     return H5_LIB
 
 
+GEN_LIB = os.path.join(HERE, "libfnet_gen.so")  # host helper of the workload generator
+
+
+def build_gen():
+    """Builds libfnet_gen.so (csrc/fnet_gen.c): the random sequential addition of the synthetic
+    cells, with FMA contraction off so that it matches the numpy restatement bit for bit."""
+    tmp = GEN_LIB + ".tmp%d" % os.getpid()
+    cmd = [CC, "-O2", "-std=c99", "-Wall", "-shared", "-fPIC", "-ffp-contract=off",
+           os.path.join(CSRC, "fnet_gen.c"), "-lm", "-o", tmp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("generator helper build failed:\n" + r.stdout)
+    os.replace(tmp, GEN_LIB)
+    return GEN_LIB
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -116,6 +132,7 @@ def build(verbose=False):
     _link([results[-1][0]], PROBE_LIB)
     build_fortran()
     build_h5()
+    build_gen()
     return LIB
 
 

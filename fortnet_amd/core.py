@@ -15,6 +15,7 @@ The classes keep the reference's data model and semantics:
 Every computation runs in libfnet_hip.so; nothing here computes descriptors or networks.
 """
 import atexit
+import copy
 import ctypes
 import math
 import weakref
@@ -234,7 +235,10 @@ class Context:
         d.activation = ptr(ns.activation)
         d.params = ptr(ns.params)
         check(self.h, self.lib.fnet_model_upload(self.h, ctypes.byref(d)), "fnet_model_upload")
-        self.model = ns
+        # the context's own record of the resident model: parameter updates (optimizer steps,
+        # upload_params) change it, never the caller's netstat
+        self.model = copy.copy(ns)
+        self.model.params = np.array(ns.params, np.float64)
 
     def upload_params(self, params):
         params = _f64(params).reshape(self.model.params.shape)
@@ -261,6 +265,43 @@ class Context:
             raise ValueError("invalid regularisation type %r" % kind)
         check(self.h, self.lib.fnet_set_regularization(self.h, float(strength), float(alpha)),
               "fnet_set_regularization")
+
+    def set_graph(self, enable=True):
+        """``fnet_set_graph``: fnet_run replays a captured HIP graph of its steady-state step."""
+        check(self.h, self.lib.fnet_set_graph(self.h, 1 if enable else 0), "fnet_set_graph")
+
+    def device_memory(self):
+        """(free, total) device memory in bytes (``fnet_device_memory``)."""
+        f, t = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        check(self.h, self.lib.fnet_device_memory(self.h, ctypes.byref(f), ctypes.byref(t)),
+              "fnet_device_memory")
+        return int(f.value), int(t.value)
+
+    def step_time(self):
+        """Device time (ms) of the last fnet_run / fnet_forces step (``fnet_step_time``)."""
+        ms = ctypes.c_double(0.0)
+        check(self.h, self.lib.fnet_step_time(self.h, ctypes.byref(ms)), "fnet_step_time")
+        return ms.value
+
+    def optimizer_init(self, kind, tol=1e-20, min_disp=1e-6, max_disp=1e4, learning_rate=0.01,
+                       memory=1000, line_search=True):
+        """``fnet_optimizer_init``: device FP64 optimizer ('sd', 'cg', 'lbfgs', 'fire') per
+        element network, started at the resident parameters."""
+        self._opt = _lib.OPTIMIZERS[kind.lower()]
+        check(self.h, self.lib.fnet_optimizer_init(self.h, self._opt, float(tol), float(min_disp),
+                                                   float(max_disp), float(learning_rate),
+                                                   int(memory), 1 if line_search else 0),
+              "fnet_optimizer_init")
+
+    def optimizer_step(self):
+        """``fnet_optimizer_step`` on the gradient of the last train call: (new params,
+        converged flag of the last network)."""
+        out = np.zeros_like(self.model.params)
+        conv = ctypes.c_int(0)
+        check(self.h, self.lib.fnet_optimizer_step(self.h, self._opt, ptr(out),
+                                                   ctypes.byref(conv)), "fnet_optimizer_step")
+        self.model.params = out
+        return out, bool(conv.value)
 
     def sd_step(self, lr, max_disp=1e4, tol=0.0):
         """``fnet_optimizer_step_sd`` on the gradient of the last train call."""
@@ -393,7 +434,15 @@ class Batch:
         check(self.ctx.h, self.lib.fnet_run(self.ctx.h, self.h), "fnet_run")
 
     def sync(self):
-        check(self.ctx.h, self.lib.fnet_sync(self.ctx.h), "fnet_sync")
+        """Waits for the batch's last step and validates it (``fnet_batch_sync``)."""
+        check(self.ctx.h, self.lib.fnet_batch_sync(self.h), "fnet_batch_sync")
+
+    def graph_replayed(self):
+        """True if the last ``run`` replayed the batch's HIP graph."""
+        g = ctypes.c_int(0)
+        check(self.ctx.h, self.lib.fnet_batch_step_info(self.h, ctypes.byref(g)),
+              "fnet_batch_step_info")
+        return bool(g.value)
 
     def stage_times(self):
         ms = np.zeros(8)

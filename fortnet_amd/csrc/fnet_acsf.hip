@@ -114,6 +114,31 @@ __device__ __forceinline__ int wave_excl_scan_i(int v, int lane) {
   return x - v;
 }
 
+// cos(theta) of an off-diagonal pair in the G4/G5 values, clamped to [-1, 1].  FP64 follows the
+// reference's value formula dot / (r_j r_k + 1e-13) (theta, acsf.F90:1159-1176; its
+// cos(acos(x)) round trip moves x by about an ulp): the regulariser lowers every cosine by a
+// relative ~1e-15, always in the same direction, and z-scores with sigma ~1e-7 turn that shift
+// into force changes of ~1e-9, so it is kept.  FP32: dot (1 / r_j) (1 / r_k).
+template <typename V>
+__device__ __forceinline__ float value_cos(const V& dj, const V& dk, float irj, float irk) {
+  const float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * irj * irk;
+  return fminf(fmaxf(c, -1.0f), 1.0f);
+}
+template <typename V>
+__device__ __forceinline__ double value_cos(const V& dj, const V& dk, double irj, double irk) {
+  const double c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) / (dj.w * dk.w + 1e-13);
+  return fmin(fmax(c, -1.0), 1.0);
+}
+
+// cos(theta) of the j == k term of a value: 1 in FP32; in FP64 the reference's
+// dot(d, d) / (r^2 + 1e-13), just below 1 (the gradient code uses 1 exactly, acsf.F90:1580-1585).
+template <typename V>
+__device__ __forceinline__ float diag_cos(const V& d, float) { return 1.0f; }
+template <typename V>
+__device__ __forceinline__ double diag_cos(const V& d, double) {
+  return (d.x * d.x + d.y * d.y + d.z * d.z) / (d.w * d.w + 1e-13);
+}
+
 struct AcsfArgs {
   const int* rowptr;  // CSR rows, or (cap > 0) row i at i * cap with cnt[i] entries
   const int* cnt;
@@ -136,7 +161,9 @@ struct AcsfArgs {
   int maxnb;
   double* G;        // [N][nG] (forward)
   const double* cG; // [N][nout][nG] (adjoint)
-  double* F;        // [N][nout][3] (adjoint)
+  double* F;        // [N][nout][3] (adjoint): the centre's own sum of its entries' gradients
+  double* gvec;     // [entries][nout][3] (adjoint): gradient of E with respect to each entry's
+                    // image position, at the entry's index in the list layout
 };
 
 // Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry, in element
@@ -445,8 +472,7 @@ __device__ __forceinline__ void pairs_fwd(const Slice<T>& sl, int lane, int A0, 
   while (it.next(ej, ek)) {
     const V dj = sl.d[ej], dk = sl.d[ek];
     const V pj = sl.pa[ej], pk = sl.pa[ek];
-    T c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
-    c = fmin(fmax(c, (T)-1), (T)1);
+    const T c = value_cos(dj, dk, pj.x, pk.x);
     T base = (SAME ? (T)2 : (T)1) * pj.y * pk.y;
     if (G4) {
       const T rx = dj.x - dk.x, ry = dj.y - dk.y, rz = dj.z - dk.z;
@@ -671,8 +697,7 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
   bool v1, v2;
   // radial-and-r_jk factor of one pair (0 when it does not exist or r_jk > rc) and its cos(theta)
   auto pair_base = [&](const V& dj, const V& pj, const V& dk, const V& pk, bool v, T& c) {
-    c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * pj.x * pk.x;
-    c = fmin(fmax(c, (T)-1), (T)1);
+    c = value_cos(dj, dk, pj.x, pk.x);
     T base = (SAME ? (T)2 : (T)1) * pj.y * pk.y;
     bool in = v;
     if (G4) {
@@ -821,10 +846,16 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
         const V pj = sl.pa[A0 + e];
         T base = pj.y * pj.y;
         if (g4) base *= pj.z * pj.z;
+        const T cjj = diag_cos(sl.d[A0 + e], (T)0);
 #pragma unroll
-        for (int l = 0; l < 2; ++l)
+        for (int l = 0; l < 2; ++l) {
+          T p = (T)1 + (l ? lam1 : lam0) * cjj;
 #pragma unroll
-          for (int m = 0; m <= MX; ++m) acc[l][m] += (double)(base * (T)grp.lpdiag[5 * l + m]);
+          for (int m = 0; m <= MX; ++m) {
+            if (m > 0) p = p * p;
+            acc[l][m] += (double)(base * p);
+          }
+        }
       }
       if (g4) pairs_fwd_lad<T, MX, MASK, true, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
       else pairs_fwd_lad<T, MX, MASK, true, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
@@ -1026,8 +1057,12 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M, PAIRS) * wave, M, PAIRS);
 
   const long long b = a.cap ? i * a.cap : (long long)a.rowptr[i];
-  const int n = a.cap ? a.cnt[i] : (int)(a.rowptr[i + 1] - b);
+  // a row longer than the slice (a list searched with a capacity from the previous step) is cut
+  // to it; the host sees the longest row and recomputes such a step
+  const int n = min(a.cap ? a.cnt[i] : (int)(a.rowptr[i + 1] - b), a.maxnb);
   const V* nd = (const V*)a.nd + b;
+  // neighbour atom of staged entry e (the adjoint keeps the row index of the entry instead)
+  auto nbr = [&](int e) -> long long { return ADJ ? (long long)a.nj[b + sl.j[e]] : sl.j[e]; };
   const T pi = (T)kPi;
   const T rcl = (T)a.rclist;
   const T pircl = pi / rcl;
@@ -1086,7 +1121,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
     if (e < n) {
       const V d = nd[e];
       sl.d[pos] = d;
-      sl.j[pos] = jj;
+      sl.j[pos] = ADJ ? e : jj;  // adjoint: the entry's index in its row, for the output
       sl.zc[pos] = key < nZ ? key : -1;
       if (!ADJ) sl.fc[pos] = (T)0.5 * (fcos(cutarg(d.w, pi, rcl, pircl)) + (T)1);
     }
@@ -1159,7 +1194,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
               if (shared) {
                 fcv = sl.fc[e];
               } else {
-                const T qj = (T)ext_q(a.ext, a.next, fn.atomid, sl.j[e]);
+                const T qj = (T)ext_q(a.ext, a.next, fn.atomid, nbr(e));
                 fcv = (T)0.5 * qi * qj * (fcos(cutarg(r, pi, rc, pirc)) + (T)1);
               }
               T term;
@@ -1216,7 +1251,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
             const T rc = (T)fn.rc;
             const T pirc = pi / rc;
             const T qq = (T)(ext_q(a.ext, a.next, fn.atomid, i) *
-                             ext_q(a.ext, a.next, fn.atomid, sl.j[e]));
+                             ext_q(a.ext, a.next, fn.atomid, nbr(e)));
             const T x = cutarg(r, pi, rc, pirc);
             fc = (T)0.5 * qq * (fcos(x) + (T)1);
             dfc = -(T)0.5 * qq * pirc * fsin(x);
@@ -1298,7 +1333,7 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
               dfc = 0;
             }
           } else {
-            qj = (T)ext_q(a.ext, a.next, grp.atomid, sl.j[e]);
+            qj = (T)ext_q(a.ext, a.next, grp.atomid, nbr(e));
             const T x = cutarg(r, pi, rc, pirc);
             fc = (T)0.5 * qi * qj * (fcos(x) + (T)1);
             dfc = -(T)0.5 * qi * qj * pirc * fsin(x);
@@ -1370,8 +1405,13 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
               const V pj = sl.pa[A0 + e];
               T base = pj.y * pj.y;
               if (g4) base *= pj.z * pj.z;
+              const T cjj = diag_cos(sl.d[A0 + e], (T)0);
 #pragma unroll
-              for (int q = 0; q < MQ; ++q) acc[q] += (double)(base * mb.pdiag[q]);
+              for (int q = 0; q < MQ; ++q) {
+                const T P = (T)1 + mb.lam[q] * cjj;
+                const T pw = ipow ? powi(P, mb.xint[q]) : fpow(P, mb.xi[q]);
+                acc[q] += (double)(base * (q < nm ? pw : (T)0));
+              }
             }
             if (g4) pairs_fwd_p<T, MQ, true, true>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
             else pairs_fwd_p<T, MQ, true, false>(ipow, sl, lane, A0, L1, B0, L2, mb, rc, eta, pi, pirc, acc);
@@ -1414,19 +1454,21 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
       }
     }
 
-    // ---- scatter of the adjoint to the forces ----
+    // ---- output of the adjoint: per-entry gradients and the centre's sum ----
+    // The neighbour side (-g of every entry whose neighbour is an atom) is gathered per atom
+    // afterwards through the mirror entries (k_force_gather), in a fixed order and without
+    // atomics; the centre side (+sum of the centre's entries) is written here.
     if (ADJ) {
-      double* F = a.F;
       double sx = 0, sy = 0, sz = 0;
       for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
         if (e >= n) break;
         const double gx = (double)sl.gs[3 * e], gy = (double)sl.gs[3 * e + 1],
                      gz = (double)sl.gs[3 * e + 2];
-        const long long jj = sl.j[e];
-        unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 0], -gx);
-        unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 1], -gy);
-        unsafeAtomicAdd(&F[(jj * a.nout + t) * 3 + 2], -gz);
+        double* gv = a.gvec + ((b + sl.j[e]) * a.nout + t) * 3;
+        gv[0] = gx;
+        gv[1] = gy;
+        gv[2] = gz;
         sx += gx;
         sy += gy;
         sz += gz;
@@ -1435,9 +1477,10 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
       sy = wave_sum(sy);
       sz = wave_sum(sz);
       if (lane == 0) {
-        unsafeAtomicAdd(&F[(i * a.nout + t) * 3 + 0], sx);
-        unsafeAtomicAdd(&F[(i * a.nout + t) * 3 + 1], sy);
-        unsafeAtomicAdd(&F[(i * a.nout + t) * 3 + 2], sz);
+        double* f = a.F + (i * a.nout + t) * 3;
+        f[0] = sx;
+        f[1] = sy;
+        f[2] = sz;
       }
       wave_sync();
     }
@@ -1501,6 +1544,7 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.G = (double*)b->G;
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
+  a.gvec = b->gvec;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
   if constexpr (ADJ && LAD) {
     if (pairs) {
@@ -1528,6 +1572,8 @@ hipError_t launch(fnet_batch* b, const DevModel& m, long long a0, long long a1, 
 }
 
 }  // namespace
+
+int acsf_slice_entries(int max_nb) { return slice_entries(max_nb); }
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj) {
   (void)nZ;

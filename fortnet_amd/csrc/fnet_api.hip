@@ -59,6 +59,8 @@ void free_model(DevModel& m) {
   dfree(m.params_f);
   dfree(m.wpack);
   m.wpack = nullptr;
+  dfree(m.bpack);
+  m.bpack = nullptr;
   m.radial = nullptr;
   m.groups = nullptr;
   m.zmean = m.zinv = nullptr;
@@ -67,6 +69,9 @@ void free_model(DevModel& m) {
   m.params_f = nullptr;
   m.valid = false;
 }
+
+// element networks in FP64 (FP64 and FP32_NET64); the others run them in FP32 or BF16
+bool net64(int precision) { return precision == FNET_FP64 || precision == FNET_FP32_NET64; }
 
 int zcode_of(const DevModel& m, int z) {
   for (int c = 0; c < m.nZ; ++c)
@@ -117,6 +122,8 @@ void ladder_form(AngGroup& a) {
 
 void invalidate(fnet_batch* b) {
   b->have_nl = b->have_G = b->have_E = b->have_F = false;
+  b->have_mirror = false;
+  b->pending = false;
 }
 
 // Packs the host copy of the parameters into MFMA fragments and uploads them (FP32 networks on
@@ -129,6 +136,15 @@ int upload_wpack(fnet_ctx* ctx, DevModel& m, hipStream_t s) {
   if (!m.wpack) HIPCHECK(ctx, dalloc(&m.wpack, pk.size()));
   HIPCHECK(ctx, hipMemcpyAsync(m.wpack, pk.data(), sizeof(float) * pk.size(),
                                hipMemcpyHostToDevice, s));
+  // BF16 fragments of the hidden-to-hidden layers (none for networks with one hidden layer)
+  std::vector<uint16_t> bk;
+  pack_bf16_weights(m.shape, m.nspecies, m.h_params.data(), bk, m.bk_stride, m.bk_fw, m.bk_rw,
+                    m.bk_sz);
+  if (!bk.empty()) {
+    if (!m.bpack) HIPCHECK(ctx, dalloc(&m.bpack, bk.size()));
+    HIPCHECK(ctx, hipMemcpyAsync(m.bpack, bk.data(), sizeof(uint16_t) * bk.size(),
+                                 hipMemcpyHostToDevice, s));
+  }
   HIPCHECK(ctx, hipStreamSynchronize(s));
   return FNET_OK;
 }
@@ -137,7 +153,7 @@ int upload_wpack(fnet_ctx* ctx, DevModel& m, hipStream_t s) {
 int ensure_tiles(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
-  const int TA = mlp_tile_atoms(m.shape, b->precision != FNET_FP32);
+  const int TA = mlp_tile_atoms(m.shape, net64(b->precision));
   const int MW = mlp_mfma_waves(m.shape);  // 16 atoms per wave on the MFMA kernel
   if (b->tile_atoms == TA && b->mtile_waves == MW && b->tile_launch == b->launch && b->tile_sp)
     return FNET_OK;
@@ -199,6 +215,7 @@ int ensure_tiles(fnet_batch* b) {
     HIPCHECK(ctx, upload(&b->mtile_cnt, mcnt.data(), mcnt.size(), s));
   }
   HIPCHECK(ctx, hipStreamSynchronize(s));
+  b->gen++;
   b->nmtiles = (int)msp.size();
   b->mtile_waves = MW;
   b->ntiles = (int)tsp.size();
@@ -221,6 +238,7 @@ int ensure_results(fnet_batch* b) {
   HIPCHECK(ctx, dalloc(&b->Estruct, (size_t)S * m.nout));
   HIPCHECK(ctx, dalloc((double**)&b->F, (size_t)N * m.nout * 3));
   HIPCHECK(ctx, dalloc(&b->lossgrad, (size_t)N * m.nout));
+  b->gen++;
   b->res_N = N;
   b->res_S = S;
   b->res_nG = m.nG;
@@ -252,6 +270,7 @@ bool ensure_padded(fnet_batch* b) {
       return false;
     }
     b->pad_len = need;
+    b->gen++;
   }
   return true;
 }
@@ -321,6 +340,7 @@ int do_neighbours(fnet_batch* b) {
   if (b->launch == FNET_BATCHED && (b->nb_cap == 0 || b->max_nb > b->nb_cap))
     b->nb_cap = ((b->max_nb + b->max_nb / 4 + 8) + 7) / 8 * 8;
   b->have_nl = true;
+  b->have_mirror = false;
   b->result_precision = b->precision;
   return FNET_OK;
 }
@@ -342,7 +362,7 @@ int do_acsf(fnet_batch* b) {
 int do_mlp(fnet_batch* b, int gradin) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
-  const int fp64 = b->precision != FNET_FP32;
+  const int fp64 = net64(b->precision);
   int rc = ensure_tiles(b);
   if (rc) return rc;
   const bool mfma = !fp64 && m.wpack && b->mtile_waves > 0;
@@ -377,18 +397,78 @@ int do_reduce(fnet_batch* b) {
   return FNET_OK;
 }
 
+// Entries of the current list layout: padded rows or CSR.
+long long layout_entries(const fnet_batch* b) {
+  return b->nl_padded ? (long long)b->N * b->nb_cap : b->npairs;
+}
+
+// Per-entry adjoint gradients and mirror entries for the current list layout.
+int ensure_force_buffers(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const long long entries = std::max<long long>(layout_entries(b), 1);
+  if (entries >= (1LL << 31))
+    return fail(ctx, FNET_ERR_CAPACITY, "neighbour list layout exceeds 2^31 entries");
+  const long long glen = entries * ctx->model.nout * 3;
+  if (entries > b->gv_entries) {
+    dfree(b->mir);
+    dfree(b->orph);
+    b->mir = b->orph = nullptr;
+    const long long cap = entries + entries / 8 + 1024;
+    HIPCHECK(ctx, dalloc(&b->mir, (size_t)cap));
+    HIPCHECK(ctx, dalloc(&b->orph, (size_t)cap));
+    b->gv_entries = cap;
+    b->gen++;
+  }
+  if (glen > b->gv_len) {
+    dfree(b->gvec);
+    b->gvec = nullptr;
+    const long long cap = glen + glen / 8 + 4096;
+    HIPCHECK(ctx, dalloc(&b->gvec, (size_t)cap));
+    b->gv_len = cap;
+    b->gen++;
+  }
+  if (!b->d_norph) HIPCHECK(ctx, dalloc(&b->d_norph, 1));
+  return FNET_OK;
+}
+
+// Mirror entries of the current list (once per list).
+int do_mirror(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  if (b->have_mirror) return FNET_OK;
+  int rc = ensure_force_buffers(b);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const long long entries = layout_entries(b);
+  if (b->launch == FNET_PER_STRUCT) {
+    for (int st = 0; st < b->S; ++st)
+      HIPCHECK(ctx, launch_mirror(b, b->h_off[st], b->h_off[st + 1], st == 0 ? entries : 0, s));
+  } else {
+    HIPCHECK(ctx, launch_mirror(b, 0, b->N, entries, s));
+  }
+  b->have_mirror = true;
+  return FNET_OK;
+}
+
+// Forces: the adjoint writes each entry's gradient and the centre's own sum, then every atom
+// gathers its neighbour side through the mirror entries (no atomics, fixed order).
 int do_adjoint(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
   const int fp64 = b->precision == FNET_FP64;
-  const size_t el = sizeof(double);
-  HIPCHECK(ctx, hipMemsetAsync(b->F, 0, el * (size_t)b->N * m.nout * 3, ctx->stream));
+  int rc = do_mirror(b);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  HIPCHECK(ctx, launch_zero_int(b->d_norph, 1, s));
   if (b->launch == FNET_PER_STRUCT) {
-    for (int st = 0; st < b->S; ++st)
-      HIPCHECK(ctx, launch_acsf_adj(b, m, b->h_off[st], b->h_off[st + 1], fp64, ctx->stream));
+    for (int st = 0; st < b->S; ++st) {
+      HIPCHECK(ctx, launch_acsf_adj(b, m, b->h_off[st], b->h_off[st + 1], fp64, s));
+      HIPCHECK(ctx, launch_force_gather(b, m.nout, b->h_off[st], b->h_off[st + 1], s));
+    }
   } else {
-    HIPCHECK(ctx, launch_acsf_adj(b, m, 0, b->N, fp64, ctx->stream));
+    HIPCHECK(ctx, launch_acsf_adj(b, m, 0, b->N, fp64, s));
+    HIPCHECK(ctx, launch_force_gather(b, m.nout, 0, b->N, s));
   }
+  HIPCHECK(ctx, launch_force_orphans(b, m.nout, s));
   b->have_F = true;
   return FNET_OK;
 }
@@ -452,6 +532,7 @@ int setup_cells(fnet_batch* b, const double* lat, const int* per) {
     HIPCHECK(ctx, dalloc(&b->cell_start, (size_t)total + 1));
     HIPCHECK(ctx, dalloc(&b->cell_scan_tmp, (size_t)scan_tmp_size(total)));
     b->ncells = total;
+    b->gen++;
   }
   b->h_cellinfo = info;
   HIPCHECK(ctx, hipMemcpyAsync(b->cellinfo, info.data(), sizeof(int) * info.size(),
@@ -460,6 +541,178 @@ int setup_cells(fnet_batch* b, const double* lat, const int* per) {
                                hipMemcpyHostToDevice, s));
   HIPCHECK(ctx, hipStreamSynchronize(s));
   return FNET_OK;
+}
+
+// The full step with the host in the loop: the neighbour search reads the list size and the
+// longest row back before the ACSF stages are sized (the first step of a batch, the per-structure
+// launches, and the redo of a steady-state step whose list outgrew its capacities).
+int run_sync_step(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  invalidate(b);
+  b->last_graph = 0;
+  ev_record(ctx, 0);
+  int rc = do_neighbours(b);
+  if (rc) return rc;
+  ev_record(ctx, 1);
+  rc = do_acsf(b);
+  if (rc) return rc;
+  ev_record(ctx, 2);
+  rc = do_mlp(b, 1);
+  if (rc) return rc;
+  ev_record(ctx, 3);
+  rc = do_adjoint(b);
+  if (rc) return rc;
+  ev_record(ctx, 4);
+  rc = do_reduce(b);
+  if (rc) return rc;
+  ev_record(ctx, 5);
+  ctx->n_ev = 6;
+  return FNET_OK;
+}
+
+// A batched step can skip every host read-back once a previous list gave the row capacity of the
+// one-pass search (nb_cap) and the longest row the ACSF slices are sized for (max_nb); buffers
+// are sized here, outside any capture.
+bool fast_step_ready(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  if (b->launch != FNET_BATCHED || b->N <= 0 || b->nb_cap <= 0 || b->max_nb <= 0) return false;
+  if (acsf_slice_entries(b->max_nb) > b->nb_cap) return false;
+  if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > 64 * 1024) return false;
+  if (!ensure_padded(b)) return false;
+  const bool was = b->nl_padded;
+  b->nl_padded = true;
+  if (ensure_force_buffers(b)) {
+    b->nl_padded = was;
+    (void)hipGetLastError();
+    return false;
+  }
+  if (!b->h_maxnb) {
+    if (hipHostMalloc((void**)&b->h_maxnb, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      b->h_maxnb = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    *b->h_maxnb = 0;
+  }
+  return true;
+}
+
+// Kernels of one steady-state step: no host read-back, no allocation and no memset, so the
+// sequence runs eagerly or as a captured HIP graph.  The padded rows are searched with capacity
+// nb_cap and every row is cut to the ACSF slice of the previous longest row; the longest row of
+// this list is reduced on the device and copied to pinned memory with the step (validate_step).
+int enqueue_fast_step(fnet_batch* b, bool events) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  hipStream_t s = ctx->stream;
+  const double rc = m.rcmax;
+  const int fp64 = b->precision == FNET_FP64;
+  if (events) ev_record(ctx, 0);
+  HIPCHECK(ctx, launch_struct_prep(b, rc, s));
+  HIPCHECK(ctx, launch_cell_bin(b, s));
+  HIPCHECK(ctx, launch_neigh_pad(b, rc, fp64, s));
+  HIPCHECK(ctx, launch_cells_pad(b, rc, fp64, s));
+  HIPCHECK(ctx, launch_row_max(b->cnt, b->N, b->d_maxnb, s));
+  HIPCHECK(ctx, launch_mirror(b, 0, b->N, layout_entries(b), s));
+  if (events) ev_record(ctx, 1);
+  HIPCHECK(ctx, launch_acsf_fwd(b, m, 0, b->N, fp64, s));
+  if (events) ev_record(ctx, 2);
+  const bool mfma = !net64(b->precision) && m.wpack && b->mtile_waves > 0;
+  if (mfma) HIPCHECK(ctx, launch_mlp_mfma(b, m, 0, b->nmtiles, 1, s));
+  else HIPCHECK(ctx, launch_mlp_fwd(b, m, 0, b->ntiles, net64(b->precision), 1, s));
+  if (events) ev_record(ctx, 3);
+  HIPCHECK(ctx, launch_acsf_adj(b, m, 0, b->N, fp64, s));
+  HIPCHECK(ctx, launch_zero_int(b->d_norph, 1, s));
+  HIPCHECK(ctx, launch_force_gather(b, m.nout, 0, b->N, s));
+  HIPCHECK(ctx, launch_force_orphans(b, m.nout, s));
+  if (events) ev_record(ctx, 4);
+  HIPCHECK(ctx, launch_energy_reduce(b, m, 0, b->S, s));
+  if (events) ev_record(ctx, 5);
+  HIPCHECK(ctx, hipMemcpyAsync(b->h_maxnb, b->d_maxnb, sizeof(int), hipMemcpyDeviceToHost, s));
+  return FNET_OK;
+}
+
+void drop_graph(fnet_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->gkey.clear();
+}
+
+// What the captured kernels' launch parameters depend on.
+std::vector<long long> graph_key(const fnet_batch* b) {
+  const fnet_ctx* ctx = b->ctx;
+  return {b->gen, ctx->model_gen, b->precision, b->launch, b->nb_cap, b->max_nb, b->ncells,
+          b->N, b->S, b->ntiles, b->nmtiles};
+}
+
+int run_fast_step(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  b->have_nl = b->have_G = b->have_E = b->have_F = false;
+  b->nl_padded = true;
+  b->last_graph = 0;
+  if (ctx->use_graph && !b->graph_failed) {
+    const std::vector<long long> key = graph_key(b);
+    if (!b->gexec || key != b->gkey) {
+      drop_graph(b);
+      HIPCHECK(ctx, hipStreamSynchronize(s));
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      int rc = e == hipSuccess ? enqueue_fast_step(b, false) : FNET_ERR_HIP;
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(s, &g);
+      if (e == hipSuccess && rc == FNET_OK && e2 == hipSuccess && g) {
+        b->graph = g;
+        if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess) b->gexec = nullptr;
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+      if (!b->gexec) {
+        // no graph for this batch: its steps run eagerly from here on
+        drop_graph(b);
+        (void)hipGetLastError();
+        b->graph_failed = 1;
+      } else {
+        b->gkey = key;
+      }
+    }
+    if (b->gexec) {
+      ev_record(ctx, 0);
+      HIPCHECK(ctx, hipGraphLaunch(b->gexec, s));
+      ev_record(ctx, 1);
+      ctx->n_ev = 2;
+      b->last_graph = 1;
+    }
+  }
+  if (!b->last_graph) {
+    const int rc = enqueue_fast_step(b, true);
+    if (rc) return rc;
+    ctx->n_ev = 6;
+  }
+  b->have_nl = b->have_G = b->have_E = b->have_F = true;
+  b->have_mirror = true;
+  b->result_precision = b->precision;
+  b->pending = true;
+  return FNET_OK;
+}
+
+// Checks the longest row of a steady-state step once it has run.  A row longer than the ACSF
+// slice (or the row capacity) was cut, so that step is recomputed on the synchronous path, which
+// also raises the capacities for the next steps; a list that shrank well below its slice lowers
+// the slice.
+int validate_step(fnet_batch* b) {
+  if (!b->pending) return FNET_OK;
+  fnet_ctx* ctx = b->ctx;
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  b->pending = false;
+  const int mx = *b->h_maxnb;
+  if (mx <= acsf_slice_entries(b->max_nb) && mx <= b->nb_cap) {
+    if (mx + 16 < b->max_nb) b->max_nb = mx;
+    return FNET_OK;
+  }
+  return run_sync_step(b);
 }
 
 int check_ready(fnet_ctx* ctx, fnet_batch* b) {
@@ -513,6 +766,9 @@ void fnet_ctx_destroy(fnet_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   free_model(ctx->model);
   dfree(ctx->d_grad);
+  dfree(ctx->opt_vecs);
+  dfree(ctx->opt_states);
+  dfree(ctx->opt_conv);
   for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -721,6 +977,7 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   m.valid = true;
   free_model(ctx->model);
   ctx->model = m;
+  ctx->model_gen++;
   ctx->have_grad = false;
   return FNET_OK;
 }
@@ -743,6 +1000,43 @@ int fnet_params_upload(fnet_ctx* ctx, const double* params) {
 }  // extern "C"
 
 namespace {
+
+// Lattice vectors of one periodic structure (column c = vector c): fnetdata.F90:1151-1154 rejects
+// dependent vectors, and the image ranges of latpointiter.F90:226-253 must stay finite and modest
+// (at most 1e6 lattice images), else the neighbour search would loop over them.
+int check_lattice(fnet_ctx* ctx, const double* L, double rcmax, int s, long long natoms) {
+  auto o = [&](int r, int c) { return L[c * 3 + r]; };
+  for (int q = 0; q < 9; ++q)
+    if (!std::isfinite(L[q]))
+      return fail(ctx, FNET_ERR_ARG, "non-finite lattice vector in structure " +
+                                         std::to_string(s));
+  const double det = o(0, 0) * (o(1, 1) * o(2, 2) - o(2, 1) * o(1, 2)) -
+                     o(0, 1) * (o(1, 0) * o(2, 2) - o(2, 0) * o(1, 2)) +
+                     o(0, 2) * (o(1, 0) * o(2, 1) - o(2, 0) * o(1, 1));
+  if (!(std::fabs(det) >= 1e-12))
+    return fail(ctx, FNET_ERR_ARG, "dependent lattice vectors in structure " + std::to_string(s));
+  double cof[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      cof[r][c] = o(r1, c1) * o(r2, c2) - o(r1, c2) * o(r2, c1);
+    }
+  double nimg = 1.0;
+  for (int a = 0; a < 3; ++a) {
+    // row a of inv(L) is column a of the cofactor matrix divided by det
+    double ss = 0.0;
+    for (int c = 0; c < 3; ++c) ss += (cof[c][a] / det) * (cof[c][a] / det);
+    nimg *= 2.0 * (std::floor(rcmax * std::sqrt(ss)) + 1.0) + 1.0;
+  }
+  if (!(nimg <= 1.0e6))
+    return fail(ctx, FNET_ERR_CAPACITY, "cell too small for the cutoff (more than 1e6 "
+                                        "lattice images) in structure " + std::to_string(s));
+  // neighbour entries carry the 32-bit key (image + 1) natoms + atom
+  if (!(nimg * (double)natoms < 2147483647.0))
+    return fail(ctx, FNET_ERR_CAPACITY, "lattice images x atoms exceed 2^31 in structure " +
+                                            std::to_string(s));
+  return FNET_OK;
+}
 
 // Validates a batch descriptor and loads it into b, reusing b's device buffers when they are
 // large enough (fnet_batch_reload) or allocating them (fnet_batch_upload).
@@ -770,32 +1064,9 @@ int load_batch(fnet_ctx* ctx, fnet_batch* b, const fnet_batch_desc* d) {
     if (per[s]) {
       if (!d->latvecs) return fail(ctx, FNET_ERR_ARG, "periodic structure without lattice vectors");
       for (int q = 0; q < 9; ++q) lat[9 * s + q] = d->latvecs[9 * s + q];
-      // fnetdata.F90:1151-1154 rejects dependent lattice vectors; the image ranges of
-      // latpointiter.F90:226-253 must also stay finite and modest
-      const double* L = &lat[9 * s];
-      auto o = [&](int r, int c) { return L[c * 3 + r]; };
-      const double det = o(0, 0) * (o(1, 1) * o(2, 2) - o(2, 1) * o(1, 2)) -
-                         o(0, 1) * (o(1, 0) * o(2, 2) - o(2, 0) * o(1, 2)) +
-                         o(0, 2) * (o(1, 0) * o(2, 1) - o(2, 0) * o(1, 1));
-      if (!(std::fabs(det) >= 1e-12))
-        return fail(ctx, FNET_ERR_ARG, "dependent lattice vectors in structure " +
-                                           std::to_string(s));
-      double cof[3][3];
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-          const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-          cof[r][c] = o(r1, c1) * o(r2, c2) - o(r1, c2) * o(r2, c1);
-        }
-      double nimg = 1.0;
-      for (int a = 0; a < 3; ++a) {
-        // row a of inv(L) is column a of the cofactor matrix divided by det
-        double ss = 0.0;
-        for (int c = 0; c < 3; ++c) ss += (cof[c][a] / det) * (cof[c][a] / det);
-        nimg *= 2.0 * (std::floor(m.rcmax * std::sqrt(ss)) + 1.0) + 1.0;
-      }
-      if (!(nimg <= 1.0e6))
-        return fail(ctx, FNET_ERR_CAPACITY, "cell too small for the cutoff (more than 1e6 "
-                                            "lattice images) in structure " + std::to_string(s));
+      const int lrc = check_lattice(ctx, &lat[9 * s], m.rcmax, s,
+                                    d->offsets[s + 1] - d->offsets[s]);
+      if (lrc) return lrc;
     }
     for (long long i = d->offsets[s]; i < d->offsets[s + 1]; ++i) astruct[i] = s;
   }
@@ -850,6 +1121,31 @@ int load_batch(fnet_ctx* ctx, fnet_batch* b, const fnet_batch_desc* d) {
   b->h_species.assign(d->species, d->species + N);
   b->h_lat = lat;
   b->h_periodic = per;
+  // chunks of the energy reduction: kEnergyChunk atoms of one structure each (the host tables
+  // live until the stream synchronisation below: the copies may read them late)
+  std::vector<int> cs, cf(S + 1, 0);
+  {
+    for (int st = 0; st < S; ++st) {
+      cf[st] = (int)cs.size();
+      const long long na = d->offsets[st + 1] - d->offsets[st];
+      for (long long q = 0; q < na; q += kEnergyChunk) cs.push_back(st);
+    }
+    cf[S] = (int)cs.size();
+    dfree(b->chunk_struct);
+    dfree(b->chunk_first);
+    b->chunk_struct = b->chunk_first = nullptr;
+    HIPCHECK(ctx, upload(&b->chunk_struct, cs.data(), cs.size(), s));
+    HIPCHECK(ctx, upload(&b->chunk_first, cf.data(), cf.size(), s));
+    const long long need = std::max<long long>((long long)cs.size(), 1) * kMaxOut;
+    if (need > b->epart_len) {
+      dfree(b->Epart);
+      b->Epart = nullptr;
+      HIPCHECK(ctx, dalloc(&b->Epart, (size_t)need));
+      b->epart_len = need;
+    }
+    b->nchunks = (int)cs.size();
+    b->h_chunk_first = cf;
+  }
   if (N) HIPCHECK(ctx, hipMemcpyAsync(b->xyz, d->coords, sizeof(double) * 3 * N,
                                       hipMemcpyHostToDevice, s));
   if (S) {
@@ -874,6 +1170,7 @@ int load_batch(fnet_ctx* ctx, fnet_batch* b, const fnet_batch_desc* d) {
   invalidate(b);
   b->tile_atoms = 0;  // species layout changed: rebuild the network tiles
   b->result_precision = -1;
+  b->gen++;
   return setup_cells(b, lat.data(), per.data());
 }
 
@@ -919,8 +1216,11 @@ void fnet_batch_destroy(fnet_batch* b) {
                   b->Eatom, b->Estruct, b->F, b->lossgrad, b->d_targets, b->cellinfo,
                   b->cell_off, b->cell_count, b->cell_start, b->cell_atoms, b->atom_cell,
                   b->atom_m, b->cell_scan_tmp, b->mtile_sp, b->mtile_start, b->mtile_cnt,
-                  b->pnj, b->pnimg, b->pnd};
+                  b->pnj, b->pnimg, b->pnd, b->gvec, b->mir, b->orph, b->d_norph,
+                  b->chunk_struct, b->chunk_first, b->Epart, b->gpart};
   for (void* p : ptrs) dfree(p);
+  drop_graph(b);
+  if (b->h_maxnb) (void)hipHostFree(b->h_maxnb);
   if (b->ctx->last_train_batch == b) b->ctx->last_train_batch = nullptr;
   delete b;
 }
@@ -929,6 +1229,14 @@ int fnet_batch_update_coords(fnet_batch* b, const double* coords, const double* 
   if (!b) return FNET_ERR_ARG;
   fnet_ctx* ctx = b->ctx;
   if (!coords) return fail(ctx, FNET_ERR_ARG, "coords is NULL");
+  // the new cells pass the checks of a batch upload before anything is copied or launched
+  if (latvecs)
+    for (int st = 0; st < b->S; ++st)
+      if (b->h_periodic[st]) {
+        const int lrc = check_lattice(ctx, latvecs + 9 * st, ctx->model.rcmax, st,
+                                      b->h_off[st + 1] - b->h_off[st]);
+        if (lrc) return lrc;
+      }
   HIPCHECK(ctx, hipSetDevice(ctx->device));
   HIPCHECK(ctx, hipMemcpyAsync(b->xyz, coords, sizeof(double) * 3 * b->N, hipMemcpyHostToDevice,
                                ctx->stream));
@@ -947,7 +1255,7 @@ int fnet_batch_update_coords(fnet_batch* b, const double* coords, const double* 
 
 int fnet_batch_set_mode(fnet_batch* b, int precision, int launch) {
   if (!b) return FNET_ERR_ARG;
-  if (precision != FNET_FP64 && precision != FNET_FP32 && precision != FNET_FP32_NET64)
+  if (precision < FNET_FP64 || precision > FNET_BF16_FAST)
     return fail(b->ctx, FNET_ERR_ARG, "unknown precision");
   if (launch != FNET_BATCHED && launch != FNET_PER_STRUCT)
     return fail(b->ctx, FNET_ERR_ARG, "unknown launch mode");
@@ -972,9 +1280,33 @@ int fnet_neighbour_list(fnet_ctx* ctx, fnet_batch* b, long long* n_pairs) {
 int fnet_get_neighbours(fnet_ctx* ctx, fnet_batch* b, long long* rowptr, int* j, int* img) {
   int rc = check_ready(ctx, b);
   if (rc) return rc;
+  rc = validate_step(b);
+  if (rc) return rc;
   if (!b->have_nl) return fail(ctx, FNET_ERR_STATE, "no neighbour list computed");
-  if (b->nl_padded)
+  if (b->nl_padded) {
+    // row offsets of the padded rows (a steady-state step does not form them), then the rows
+    // copied to them
+    HIPCHECK(ctx, launch_scan(b->cnt, b->rowptr, b->N, b->scan_tmp, ctx->stream, nullptr));
+    int P = 0;
+    HIPCHECK(ctx, hipMemcpyAsync(&P, b->rowptr + b->N, sizeof(int), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (P > b->pcap) {
+      dfree(b->nj);
+      dfree(b->nimg);
+      dfree(b->nd);
+      b->nj = b->nimg = nullptr;
+      b->nd = nullptr;
+      const long long cap = (long long)P + P / 8 + 1024;
+      HIPCHECK(ctx, dalloc(&b->nj, cap));
+      HIPCHECK(ctx, dalloc(&b->nimg, cap));
+      HIPCHECK(ctx, dalloc((double4**)&b->nd, cap));
+      b->pcap = cap;
+      b->gen++;
+    }
+    b->npairs = P;
     HIPCHECK(ctx, launch_compact_rows(b, b->result_precision == FNET_FP64, ctx->stream));
+  }
   std::vector<int> rp(b->N + 1);
   HIPCHECK(ctx, hipMemcpyAsync(rp.data(), b->rowptr, sizeof(int) * (b->N + 1),
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -987,11 +1319,19 @@ int fnet_get_neighbours(fnet_ctx* ctx, fnet_batch* b, long long* rowptr, int* j,
   HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (rowptr)
     for (long long i = 0; i <= b->N; ++i) rowptr[i] = rp[i];
+  // the device list stores each entry's key (image + 1) n + (j - i) mod n: image = key / n - 1
+  if (img)
+    for (int st = 0; st < b->S; ++st) {
+      const int n = (int)(b->h_off[st + 1] - b->h_off[st]);
+      for (long long e = rp[b->h_off[st]]; e < rp[b->h_off[st + 1]]; ++e) img[e] = img[e] / n - 1;
+    }
   return FNET_OK;
 }
 
 int fnet_acsf_forward(fnet_ctx* ctx, fnet_batch* b, double* G_out) {
   int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  rc = validate_step(b);
   if (rc) return rc;
   if (!b->have_nl || b->result_precision != b->precision) {
     rc = do_neighbours(b);
@@ -1006,6 +1346,8 @@ int fnet_acsf_forward(fnet_ctx* ctx, fnet_batch* b, double* G_out) {
 
 int fnet_energy(fnet_ctx* ctx, fnet_batch* b, double* E_struct, double* E_atom) {
   int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  rc = validate_step(b);
   if (rc) return rc;
   if (!b->have_G || b->result_precision != b->precision) {
     rc = fnet_acsf_forward(ctx, b, nullptr);
@@ -1031,6 +1373,8 @@ int fnet_energy(fnet_ctx* ctx, fnet_batch* b, double* E_struct, double* E_atom) 
 int fnet_forces(fnet_ctx* ctx, fnet_batch* b, double* F) {
   int rc = check_ready(ctx, b);
   if (rc) return rc;
+  rc = validate_step(b);
+  if (rc) return rc;
   if (!b->have_G || b->result_precision != b->precision) {
     rc = fnet_acsf_forward(ctx, b, nullptr);
     if (rc) return rc;
@@ -1051,26 +1395,42 @@ int fnet_forces(fnet_ctx* ctx, fnet_batch* b, double* F) {
 int fnet_run(fnet_ctx* ctx, fnet_batch* b) {
   int rc = check_ready(ctx, b);
   if (rc) return rc;
-  invalidate(b);
   rc = ensure_tiles(b);
   if (rc) return rc;
-  ev_record(ctx, 0);
-  rc = do_neighbours(b);
-  if (rc) return rc;
-  ev_record(ctx, 1);
-  rc = do_acsf(b);
-  if (rc) return rc;
-  ev_record(ctx, 2);
-  rc = do_mlp(b, 1);
-  if (rc) return rc;
-  ev_record(ctx, 3);
-  rc = do_adjoint(b);
-  if (rc) return rc;
-  ev_record(ctx, 4);
-  rc = do_reduce(b);
-  if (rc) return rc;
-  ev_record(ctx, 5);
-  ctx->n_ev = 6;
+  if (fast_step_ready(b)) return run_fast_step(b);
+  return run_sync_step(b);
+}
+
+int fnet_device_memory(fnet_ctx* ctx, long long* free_bytes, long long* total_bytes) {
+  if (!ctx || !free_bytes || !total_bytes) return FNET_ERR_ARG;
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  HIPCHECK(ctx, hipMemGetInfo(&f, &t));
+  *free_bytes = (long long)f;
+  *total_bytes = (long long)t;
+  return FNET_OK;
+}
+
+int fnet_set_graph(fnet_ctx* ctx, int enable) {
+  if (!ctx) return FNET_ERR_ARG;
+  ctx->use_graph = enable ? 1 : 0;
+  return FNET_OK;
+}
+
+int fnet_step_time(fnet_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return FNET_ERR_ARG;
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *ms = 0.0;
+  if (ctx->n_ev < 2) return FNET_OK;
+  float t = 0.f;
+  HIPCHECK(ctx, hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[ctx->n_ev - 1]));
+  *ms = t;
+  return FNET_OK;
+}
+
+int fnet_batch_step_info(const fnet_batch* b, int* graph) {
+  if (!b || !graph) return FNET_ERR_ARG;
+  *graph = b->last_graph;
   return FNET_OK;
 }
 
@@ -1091,6 +1451,8 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
                          const double* dp_weights, double* grad, double* pred,
                          double* loss) {
   int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  rc = validate_step(b);
   if (rc) return rc;
   const DevModel& m = ctx->model;
   if (n_global < 0 || n_global > m.nout) return fail(ctx, FNET_ERR_ARG, "invalid n_global");
@@ -1144,8 +1506,16 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
     HIPCHECK(ctx, dalloc(&ctx->d_grad, (size_t)np + m.nspecies));
     ctx->grad_cap = np;
   }
-  HIPCHECK(ctx, hipMemsetAsync(ctx->d_grad, 0, sizeof(double) * np, s));
-  HIPCHECK(ctx, launch_mlp_grad(b, m, b->precision != FNET_FP32, ctx->d_grad, s));
+  {
+    const long long need = std::max<long long>((long long)b->ntiles, 1) * m.shape.nparams;
+    if (need > b->gpart_len) {
+      dfree(b->gpart);
+      b->gpart = nullptr;
+      HIPCHECK(ctx, dalloc(&b->gpart, (size_t)need));
+      b->gpart_len = need;
+    }
+  }
+  HIPCHECK(ctx, launch_mlp_grad(b, m, net64(b->precision), ctx->d_grad, s));
   if (ctx->reg_strength > 0.0)
     HIPCHECK(ctx, launch_regularize(m, ctx->d_grad, ctx->reg_strength / wsum, ctx->reg_alpha, s));
   double* dloss = dpw + b->S;
@@ -1156,9 +1526,11 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   if (pred)
     HIPCHECK(ctx, hipMemcpyAsync(pred, b->Eatom, sizeof(double) * b->N * m.nout,
                                  hipMemcpyDeviceToHost, s));
-  if (loss)
-    HIPCHECK(ctx, hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, s));
+  double lv = 0.0;
+  HIPCHECK(ctx, hipMemcpyAsync(&lv, dloss, sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHECK(ctx, hipStreamSynchronize(s));
+  if (loss) *loss = lv;
+  ctx->last_loss = lv;
   ctx->have_grad = true;
   ctx->last_train_batch = b;
   return FNET_OK;
@@ -1214,10 +1586,99 @@ int fnet_optimizer_step_sd(fnet_ctx* ctx, double lr, double max_disp, double tol
   return FNET_OK;
 }
 
+int fnet_optimizer_init(fnet_ctx* ctx, int kind, double tol, double min_disp, double max_disp,
+                        double learning_rate, int memory, int line_search) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "no model uploaded");
+  if (kind < FNET_OPT_SD || kind > FNET_OPT_FIRE) return fail(ctx, FNET_ERR_ARG, "unknown optimizer");
+  if (!(max_disp > 0.0)) return fail(ctx, FNET_ERR_ARG, "max_disp must be positive");
+  (void)min_disp;  // the reference's MinDisplacement does not enter these updates
+  const int mem = kind == FNET_OPT_LBFGS ? std::max(1, memory) : 1;
+  DevModel& m = ctx->model;
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  const long long n = m.shape.nparams;
+  const long long vstride = opt_vec_doubles(n, mem);
+  const long long need = vstride * m.nspecies;
+  if (need > ctx->opt_cap || !ctx->opt_states) {
+    dfree(ctx->opt_vecs);
+    dfree(ctx->opt_states);
+    dfree(ctx->opt_conv);
+    ctx->opt_vecs = nullptr;
+    ctx->opt_states = nullptr;
+    ctx->opt_conv = nullptr;
+    HIPCHECK(ctx, dalloc(&ctx->opt_vecs, (size_t)need));
+    HIPCHECK(ctx, dalloc(&ctx->opt_states, 64));
+    HIPCHECK(ctx, dalloc(&ctx->opt_conv, 64));
+    ctx->opt_cap = need;
+  }
+  OptState st{};
+  st.kind = kind;
+  st.converged = 0;
+  st.mem = mem;
+  st.line_search = line_search ? 1 : 0;
+  st.tol = tol;
+  st.max_disp = max_disp;
+  st.lr = learning_rate;
+  // the line minimiser of CG takes 10 iterations and tolerance 1e4, that of L-BFGS 10 and tol
+  st.lm_maxiter = 10;
+  st.lm_tol = kind == FNET_OPT_CG ? 10000.0 : tol;
+  st.cg_state = 1;
+  st.lb_it = 0;
+  st.f_it = 0;
+  st.f_reset = 0;
+  st.f_a = 0.1;                 // a_start
+  st.f_dt = 0.1 * max_disp;     // dt_init = 0.1 dt_max
+  std::vector<OptState> h(m.nspecies, st);
+  hipStream_t s = ctx->stream;
+  HIPCHECK(ctx, hipMemcpyAsync(ctx->opt_states, h.data(), sizeof(OptState) * h.size(),
+                               hipMemcpyHostToDevice, s));
+  HIPCHECK(ctx, launch_opt_init(m.params, ctx->opt_vecs, n, vstride, mem, m.nspecies, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  ctx->opt_kind = kind;
+  ctx->opt_vstride = vstride;
+  return FNET_OK;
+}
+
+int fnet_optimizer_step(fnet_ctx* ctx, int kind, double* params_out, int* converged) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "no model uploaded");
+  if (ctx->opt_kind < 0) return fail(ctx, FNET_ERR_STATE, "optimizer not initialised");
+  if (kind != ctx->opt_kind)
+    return fail(ctx, FNET_ERR_ARG, "optimizer kind differs from fnet_optimizer_init");
+  if (!ctx->have_grad) return fail(ctx, FNET_ERR_STATE, "no gradient computed");
+  DevModel& m = ctx->model;
+  HIPCHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const long long n = m.shape.nparams, np = n * m.nspecies;
+  HIPCHECK(ctx, launch_opt_step(ctx->opt_states, m.params, ctx->d_grad, ctx->opt_vecs, n,
+                                ctx->opt_vstride, ctx->last_loss, ctx->opt_conv, m.nspecies, s));
+  HIPCHECK(ctx, launch_to_float(m.params, m.params_f, np, s));
+  std::vector<int> conv(m.nspecies, 0);
+  HIPCHECK(ctx, hipMemcpyAsync(m.h_params.data(), m.params, sizeof(double) * np,
+                               hipMemcpyDeviceToHost, s));
+  HIPCHECK(ctx, hipMemcpyAsync(conv.data(), ctx->opt_conv, sizeof(int) * m.nspecies,
+                               hipMemcpyDeviceToHost, s));
+  HIPCHECK(ctx, hipStreamSynchronize(s));
+  {
+    const int prc = upload_wpack(ctx, m, s);
+    if (prc) return prc;
+  }
+  if (params_out) std::memcpy(params_out, m.h_params.data(), sizeof(double) * np);
+  if (converged) *converged = conv[m.nspecies - 1];
+  ctx->have_grad = false;
+  return FNET_OK;
+}
+
 int fnet_sync(fnet_ctx* ctx) {
   if (!ctx) return FNET_ERR_ARG;
   HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
   return FNET_OK;
+}
+
+int fnet_batch_sync(fnet_batch* b) {
+  if (!b) return FNET_ERR_ARG;
+  HIPCHECK(b->ctx, hipSetDevice(b->ctx->device));
+  return validate_step(b);
 }
 
 }  // extern "C"

@@ -214,7 +214,6 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
         key[q] = k;
         slot[q] = q;
         pj[q] = (int)(o + lj);
-        pimg[q] = img;
         V v;
         v.x = nd0;
         v.y = nd1;
@@ -262,7 +261,7 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
   for (int e = lane; e < mw; e += 64) {
     const int q = slot[e];
     a.nj[rb + e] = pj[q];
-    a.nimg[rb + e] = pimg[q];
+    a.nimg[rb + e] = (int)key[e];  // the entry's key (image + 1) n + (j - i) mod n
     out[rb + e] = pnd[q];
   }
 }
@@ -313,13 +312,14 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
 
 hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s) {
   if (b->ncells == 0 || b->N == 0) return hipSuccess;
-  FNET_HIP_TRY(hipMemsetAsync(b->cell_count, 0, sizeof(int) * b->ncells, s));
+  // fill kernels rather than memsets: this sequence is also captured into a HIP graph
+  FNET_HIP_TRY(launch_zero_int(b->cell_count, (int)b->ncells, s));
   const unsigned grid = (unsigned)((b->N + 255) / 256);
   k_cell_bin<<<grid, 256, 0, s>>>(b->xyz, b->atom_struct, b->sinfo, b->cellinfo, b->cell_off, 0,
                                   b->N, b->atom_cell, b->atom_m, b->cell_count);
   FNET_HIP_TRY(hipGetLastError());
   FNET_HIP_TRY(launch_scan(b->cell_count, b->cell_start, b->ncells, b->cell_scan_tmp, s, nullptr));
-  FNET_HIP_TRY(hipMemsetAsync(b->cell_count, 0, sizeof(int) * b->ncells, s));
+  FNET_HIP_TRY(launch_zero_int(b->cell_count, (int)b->ncells, s));
   k_cell_scatter<<<grid, 256, 0, s>>>(b->atom_struct, b->cellinfo, 0, b->N, b->atom_cell,
                                       b->cell_start, b->cell_count, b->cell_atoms);
   return hipGetLastError();

@@ -1,0 +1,209 @@
+// Force assembly without atomics: mirror entries of the neighbour list and the per-atom gather.
+//
+// forceAnalysis_analytical (forces.F90:400-413) contracts dE/dG with dG/dr, where the derivative of
+// centre i's functions moves both the centre (+) and each neighbour image (-).  The adjoint pass
+// (fnet_acsf.hip) leaves g_e = dE/d(image position of entry e) for every entry e = (i, j, n) of
+// row i and writes the centre's own sum.  The neighbour side of atom i is the sum of g over the
+// entries whose neighbour is i; in a list built with one cutoff those are exactly the mirror
+// entries (j, i, -n) of row i's own entries.  So every atom gathers -g over the mirrors of its
+// own row, in row order: the result does not depend on scheduling, and the force array is read
+// and written once per atom instead of taking three FP64 atomics per entry.
+//
+// The mirror of entry (i, j, n) is found in row j by binary search.  Rows are in the reference's
+// order (dynneighlist.F90:233-320): the central cell first, then the lattice points in
+// lexicographic order, and within each image the atoms from the centre on, so
+// key = (image + 1) * n_struct + (atom - centre) mod n_struct increases strictly along a row.
+// Image ranges are symmetric (latpointiter.F90:226-253), so the image index of -n is
+// (number of images - 1) - k for image index k (the origin skipped).
+//
+// The neighbour kernels store this key with each entry (the image index is key / n - 1).
+//
+// The distance test is evaluated as coords(j) + L n - coords(i) in FP64, whose mirror expression
+// rounds differently; a pair exactly at the cutoff can therefore be in one row and not in the
+// other.  Such an entry has no mirror: the gather puts it on an orphan list and its -g is added to
+// its neighbour's force afterwards (k_force_orphans, the only atomic update left, for pairs within
+// an ulp of the cutoff).
+#include "fnet_internal.hpp"
+
+namespace fnet {
+namespace {
+
+struct RowLayout {
+  const int* rowptr;  // CSR rows (cap == 0)
+  const int* cnt;     // padded rows: row i at i * cap with cnt[i] entries
+  int cap;
+  __device__ __forceinline__ long long base(long long i) const {
+    return cap ? i * (long long)cap : (long long)rowptr[i];
+  }
+  __device__ __forceinline__ int len(long long i) const {
+    return cap ? min(cnt[i], cap) : rowptr[i + 1] - rowptr[i];
+  }
+};
+
+__device__ __forceinline__ int mod_n(int x, int n) {
+  x %= n;
+  return x < 0 ? x + n : x;
+}
+
+// One wave per centre; lanes over the centre's entries.  The entries store their keys
+// (image + 1) n + (j - i) mod n.  Each mirror pair is searched once, from the entry with i < j (or,
+// for the centre's own images, from the one with the lower image index), which writes both
+// directions; entries left at -1 (filled before) have no mirror.
+__global__ void __launch_bounds__(256)
+    k_mirror(RowLayout rl, const int* __restrict__ nj, const int* __restrict__ key,
+             const long long* __restrict__ off, const int* __restrict__ atom_struct,
+             const StructInfo* __restrict__ sinfo, long long a0, long long a1,
+             int* __restrict__ mir) {
+  const int lane = threadIdx.x & 63;
+  const long long i = a0 + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  if (i >= a1) return;
+  const int s = atom_struct[i];
+  const long long o = off[s];
+  const int n = (int)(off[s + 1] - o);
+  const StructInfo& si = sinfo[s];
+  const int nimages =
+      si.periodic ? (si.hi[0] - si.lo[0] + 1) * (si.hi[1] - si.lo[1] + 1) *
+                            (si.hi[2] - si.lo[2] + 1) - 1
+                  : 0;
+  const int li = (int)(i - o);
+  const long long bi = rl.base(i);
+  const int len = rl.len(i);
+  for (int e = lane; e < len; e += 64) {
+    const int k = key[bi + e];
+    const int j = nj[bi + e];
+    const int lj = j - (int)o;
+    const int img = k / n - 1;
+    const int t = k - (img + 1) * n;  // (j - i) mod n
+    const int mimg = img < 0 ? -1 : nimages - 1 - img;
+    if (!(li < lj || (li == lj && img < mimg))) continue;
+    const int target = (mimg + 1) * n + (t == 0 ? 0 : n - t);
+    const long long bj = rl.base(j);
+    const int lenj = rl.len(j);
+    int lo = 0, hi = lenj;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (key[bj + mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < lenj && key[bj + lo] == target) {
+      mir[bi + e] = (int)(bj + lo);
+      mir[bj + lo] = (int)(bi + e);
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// F_i += -sum over the mirrors of row i of g, per output, in row order; entries of row i without
+// a mirror go to the orphan list.
+__global__ void __launch_bounds__(256)
+    k_force_gather(RowLayout rl, const int* __restrict__ mir, const double* __restrict__ gvec,
+                   long long a0, long long a1, int nout, double* __restrict__ F,
+                   int* __restrict__ orph, int* __restrict__ norph) {
+  const int lane = threadIdx.x & 63;
+  const long long i = a0 + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  if (i >= a1) return;
+  const long long bi = rl.base(i);
+  const int len = rl.len(i);
+  for (int e = lane; e < len; e += 64)
+    if (mir[bi + e] < 0) orph[atomicAdd(norph, 1)] = (int)(bi + e);
+  for (int t = 0; t < nout; ++t) {
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (int e = lane; e < len; e += 64) {
+      const int m = mir[bi + e];
+      if (m >= 0) {
+        const double* g = gvec + ((long long)m * nout + t) * 3;
+        x -= g[0];
+        y -= g[1];
+        z -= g[2];
+      }
+    }
+    x = wave_sum(x);
+    y = wave_sum(y);
+    z = wave_sum(z);
+    if (lane == 0) {
+      double* f = F + (i * nout + t) * 3;
+      f[0] += x;
+      f[1] += y;
+      f[2] += z;
+    }
+  }
+}
+
+// Entries without a mirror: -g to the neighbour's force.
+__global__ void __launch_bounds__(256)
+    k_force_orphans(const int* __restrict__ norph, const int* __restrict__ orph,
+                    const int* __restrict__ nj, const double* __restrict__ gvec, int nout,
+                    double* __restrict__ F) {
+  const int cnt = *norph;
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const long long e = orph[q];
+    const long long j = nj[e];
+    for (int t = 0; t < nout; ++t)
+      for (int c = 0; c < 3; ++c)
+        unsafeAtomicAdd(&F[(j * nout + t) * 3 + c], -gvec[(e * nout + t) * 3 + c]);
+  }
+}
+
+__global__ void k_zero_int(int* __restrict__ p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
+__global__ void k_fill_int(int* __restrict__ p, long long n, int v) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+RowLayout layout_of(const fnet_batch* b) {
+  RowLayout rl;
+  rl.rowptr = b->rowptr;
+  rl.cnt = b->cnt;
+  rl.cap = b->nl_padded ? b->nb_cap : 0;
+  return rl;
+}
+
+}  // namespace
+
+hipError_t launch_zero_int(int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_zero_int<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+  return hipGetLastError();
+}
+
+// Mirror entries of atoms [a0, a1); `entries` > 0 first fills the whole mirror array with -1
+// (a range of atoms whose rows are alone in their structures may skip it after the first).
+hipError_t launch_mirror(fnet_batch* b, long long a0, long long a1, long long entries,
+                         hipStream_t s) {
+  if (entries > 0)
+    k_fill_int<<<(unsigned)((entries + 255) / 256), 256, 0, s>>>(b->mir, entries, -1);
+  const long long n = a1 - a0;
+  if (n <= 0) return hipGetLastError();
+  const int* nj = b->nl_padded ? b->pnj : b->nj;
+  const int* key = b->nl_padded ? b->pnimg : b->nimg;
+  k_mirror<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(layout_of(b), nj, key, b->off, b->atom_struct,
+                                                   b->sinfo, a0, a1, b->mir);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_gather(fnet_batch* b, int nout, long long a0, long long a1,
+                               hipStream_t s) {
+  const long long n = a1 - a0;
+  if (n <= 0) return hipSuccess;
+  k_force_gather<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(layout_of(b), b->mir, b->gvec, a0, a1,
+                                                         nout, (double*)b->F, b->orph,
+                                                         b->d_norph);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_orphans(fnet_batch* b, int nout, hipStream_t s) {
+  const int* nj = b->nl_padded ? b->pnj : b->nj;
+  k_force_orphans<<<1, 256, 0, s>>>(b->d_norph, b->orph, nj, b->gvec, nout, (double*)b->F);
+  return hipGetLastError();
+}
+
+}  // namespace fnet

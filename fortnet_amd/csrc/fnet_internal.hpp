@@ -76,6 +76,20 @@ struct StructInfo {
   int pad;
 };
 
+// Scalar state of one element network's optimizer (fnet_optim.hip); the vectors live in
+// fnet_ctx::opt_vecs.
+struct OptState {
+  int kind, converged, mem, line_search;
+  double tol, max_disp, lr;
+  // line minimiser (linemin.F90)
+  int lm_state, lm_it, lm_conv, lm_maxiter;
+  double lm_tol, lm_xx[2], lm_dx[2], lm_xcur, lm_first, lm_maxx;
+  int cg_state;        // conjugate gradients: 1 before the first direction
+  int lb_it;           // L-BFGS: directions computed so far
+  int f_it, f_reset;   // FIRE
+  double f_a, f_dt;
+};
+
 // Device copy of the model.
 struct DevModel {
   int nG = 0, nR = 0, nAG = 0, nZ = 0;
@@ -101,6 +115,10 @@ struct DevModel {
   float* wpack = nullptr;
   long long pk_stride = 0;
   long long pk_fw[kMaxLayers] = {}, pk_rw[kMaxLayers] = {}, pk_b[kMaxLayers] = {};
+  // BF16 hi / lo fragments of the hidden-to-hidden layers (BF16 modes), in units of 8 values
+  uint16_t* bpack = nullptr;
+  long long bk_stride = 0;
+  long long bk_fw[kMaxLayers] = {}, bk_rw[kMaxLayers] = {}, bk_sz[kMaxLayers] = {};
   std::vector<double> h_params;
   std::vector<int> h_species_z;
   bool valid = false;
@@ -128,6 +146,15 @@ struct fnet_ctx {
   bool have_grad = false;
   fnet_batch* last_train_batch = nullptr;
   int loss = 0;               // fnet_loss of the training gradient and loss value
+  int use_graph = 0;          // fnet_run replays a captured HIP graph of its steady-state step
+  double last_loss = 0.0;     // loss of the last training-gradient call
+  // device optimizers (fnet_optimizer_init / _step)
+  int opt_kind = -1;
+  fnet::OptState* opt_states = nullptr;  // [nspecies]
+  double* opt_vecs = nullptr;            // [nspecies][opt_vstride]
+  int* opt_conv = nullptr;               // [nspecies]
+  long long opt_vstride = 0, opt_cap = 0;
+  long long model_gen = 0;    // bumped by every model upload (new device tables)
   double reg_strength = 0.0;  // elastic-net regularisation of the weight gradients
   double reg_alpha = 0.0;
 };
@@ -210,6 +237,36 @@ struct fnet_batch {
   double* Estruct = nullptr;   // [S][nout]
   void* F = nullptr;           // [N][nout][3] FP64
   double* lossgrad = nullptr;  // [N][nout]
+  // force assembly (fnet_gather.hip): per-entry adjoint gradients and mirror entries, sized for
+  // the list layout (padded rows or CSR)
+  double* gvec = nullptr;      // [entries][nout][3]
+  int* mir = nullptr;          // [entries] mirror entry (j, i, -n) of each entry, -1 if none
+  int* orph = nullptr;         // [entries] entries without a mirror
+  int* d_norph = nullptr;      // number of them
+  long long gv_entries = 0;    // allocated entries of mir / orph
+  long long gv_len = 0;        // allocated doubles of gvec
+  bool have_mirror = false;
+  // energy reduction in fixed-size chunks of atoms (two passes, fixed order)
+  int* chunk_struct = nullptr; // [nchunks]
+  int* chunk_first = nullptr;  // [S+1]
+  double* Epart = nullptr;     // [nchunks][nout]
+  std::vector<int> h_chunk_first;
+  int nchunks = 0;
+  long long epart_len = 0;
+  // weight gradient: one FP64 partial vector per network tile, reduced in tile order
+  double* gpart = nullptr;
+  long long gpart_len = 0;
+  // steady-state step of fnet_run (no host read-back): the longest row of the list it built is
+  // copied to pinned host memory with the step and checked when results are next read
+  int* h_maxnb = nullptr;
+  bool pending = false;
+  long long gen = 0;            // bumped whenever a buffer the step uses is reallocated
+  // HIP graph of the steady-state step and the launch parameters it was captured with
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey;
+  int graph_failed = 0;         // capture or instantiation failed once: run the step eagerly
+  int last_graph = 0;           // the last fnet_run replayed the graph
   double* d_targets = nullptr; // staging of training inputs
   long long targets_cap = 0;
   bool have_nl = false, have_G = false, have_E = false, have_F = false;
@@ -235,6 +292,8 @@ hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s);
 hipError_t launch_compact_rows(fnet_batch* b, int fp64, hipStream_t s);
 hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s,
                        int* maxout);
+// *out = max(cnt[0..n)) (graph-safe: zeroed by a kernel)
+hipError_t launch_row_max(const int* cnt, long long n, int* out, hipStream_t s);
 long long scan_tmp_size(long long n);
 
 // launchers (fnet_cells.hip)
@@ -243,6 +302,22 @@ hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s);
 hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
                              hipStream_t s);
+
+// launchers (fnet_optim.hip)
+long long opt_vec_doubles(long long n, int mem);
+hipError_t launch_opt_init(const double* params, double* vecs, long long n, long long vstride,
+                           int mem, int nspecies, hipStream_t s);
+hipError_t launch_opt_step(OptState* states, double* params, const double* grad, double* vecs,
+                           long long n, long long vstride, double fx, int* conv, int nspecies,
+                           hipStream_t s);
+
+// launchers (fnet_gather.hip)
+hipError_t launch_zero_int(int* p, int n, hipStream_t s);
+hipError_t launch_mirror(fnet_batch* b, long long a0, long long a1, long long fill_entries,
+                         hipStream_t s);
+hipError_t launch_force_gather(fnet_batch* b, int nout, long long a0, long long a1,
+                               hipStream_t s);
+hipError_t launch_force_orphans(fnet_batch* b, int nout, hipStream_t s);
 
 // launchers (fnet_acsf.hip)
 hipError_t launch_acsf_fwd(fnet_batch* b, const DevModel& m, long long a0, long long a1, int fp64,
@@ -253,10 +328,12 @@ hipError_t launch_acsf_adj(fnet_batch* b, const DevModel& m, long long a0, long 
 // launchers (fnet_mlp.hip)
 hipError_t launch_mlp_fwd(fnet_batch* b, const DevModel& m, int first_tile, int ntiles, int fp64,
                           int want_grad_in, hipStream_t s);
+constexpr int kEnergyChunk = 2048;  // atoms per block of the energy reduction
 hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1, hipStream_t s);
 hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                            const double* at, const double* aw, const double* dpw, double inv_wsum,
                            int loss, hipStream_t s);
+// weight gradient: per-tile partials into b->gpart, then their sum in tile order into grad
 hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad, hipStream_t s);
 hipError_t launch_loss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                        const double* at, const double* aw, const double* dpw, int loss,
@@ -274,11 +351,15 @@ int mlp_mfma_waves(const NetShape& sh);  // waves per block, 0 if the shape is n
 void pack_mfma_weights(const NetShape& sh, int nspecies, const double* params,
                        std::vector<float>& out, long long& stride, long long* fw, long long* rw,
                        long long* pb);
+void pack_bf16_weights(const NetShape& sh, int nspecies, const double* params,
+                       std::vector<uint16_t>& out, long long& stride, long long* bfw,
+                       long long* brw, long long* bsz);
 hipError_t launch_mlp_mfma(fnet_batch* b, const DevModel& m, int first_tile, int ntiles,
                            int want_grad_in, hipStream_t s);
 hipError_t launch_to_float(const double* in, float* out, long long n, hipStream_t s);
 
 size_t lds_bytes_acsf(int max_nb, int nZ, int fp64, int adj);
+int acsf_slice_entries(int max_nb);  // entries of one wave's LDS slice for rows up to max_nb
 
 }  // namespace fnet
 

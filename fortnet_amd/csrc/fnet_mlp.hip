@@ -201,7 +201,10 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpArgs a) {
   }
 }
 
-// Weight gradient of one tile (bprop), reduced over the tile in LDS and added in FP64.
+// Weight gradient of one tile (bprop), summed over the tile's atoms in T and written as the
+// tile's FP64 partial vector (a.grad + tile * nparams); k_grad_reduce adds the partials of each
+// species in tile order.  Entries that receive no gradient (the input layer's biases and the
+// dummy output weights, network.F90:397-427) are written as zero.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -209,7 +212,9 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
   const int sp = a.tile_sp[tile], start = a.tile_start[tile], cnt = a.tile_cnt[tile];
   const int nl = a.sh.nl;
   const T* P = (const T*)a.params + (long long)sp * a.sh.nparams;
-  double* g = a.grad + (long long)sp * a.sh.nparams;
+  double* g = a.grad + (long long)tile * a.sh.nparams;
+  for (int k = threadIdx.x; k < a.sh.dims[nl - 1]; k += blockDim.x) g[a.sh.woff[nl - 1] + k] = 0.0;
+  for (int k = threadIdx.x; k < a.sh.dims[0]; k += blockDim.x) g[a.sh.boff[0] + k] = 0.0;
   const int act = a.sh.act[sp];
   T* acts[kMaxLayers];
   T* fp[kMaxLayers];
@@ -245,13 +250,13 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
     for (int o = threadIdx.x; o < dout; o += blockDim.x) {
       T s = 0;
       for (int at = 0; at < cnt; ++at) s += dcur[at * dout + o];
-      unsafeAtomicAdd(&g[a.sh.boff[l] + o], (double)s);
+      g[a.sh.boff[l] + o] = (double)s;
     }
     for (int idx = threadIdx.x; idx < din * dout; idx += blockDim.x) {
       const int k = idx % din, o = idx / din;
       T s = 0;
       for (int at = 0; at < cnt; ++at) s += acts[l - 1][at * din + k] * dcur[at * dout + o];
-      unsafeAtomicAdd(&g[a.sh.woff[l - 1] + (long long)o * din + k], (double)s);
+      g[a.sh.woff[l - 1] + (long long)o * din + k] = (double)s;
     }
     if (l == 1) break;
     const T* W = P + a.sh.woff[l - 1];
@@ -268,16 +273,38 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
   }
 }
 
-// One block per structure: deterministic FP64 sum of the atomic outputs (predictBatch sums,
-// bpnn.F90:1040-1047; forces.F90:381-385).  Thread t sums atoms t, t + 256, ... in order, then the
-// four wave sums are combined in a fixed order, so the result does not depend on scheduling.
-__global__ void __launch_bounds__(256) k_energy_reduce(const double* __restrict__ Eatom,
-                                                      const long long* __restrict__ off, int s0,
-                                                      int nout, double* __restrict__ Estruct) {
+// grad[sp][k] = sum over the tiles of species sp, in tile order, of the tiles' partials.
+__global__ void k_grad_reduce(const double* __restrict__ gpart, const int* __restrict__ tile_sp,
+                              int ntiles, long long nparams, int nspecies,
+                              double* __restrict__ grad) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nparams * nspecies) return;
+  const int sp = (int)(q / nparams);
+  const long long k = q - (long long)sp * nparams;
+  double v = 0.0;
+  for (int t = 0; t < ntiles; ++t)
+    if (tile_sp[t] == sp) v += gpart[(long long)t * nparams + k];
+  grad[q] = v;
+}
+
+// Per-structure sums of the atomic outputs (predictBatch sums, bpnn.F90:1040-1047;
+// forces.F90:381-385) in FP64, in two passes with a fixed order: each block sums one chunk of
+// kEnergyChunk atoms of one structure (thread t takes atoms t, t + 256, ... in order; the four
+// wave sums combine in a fixed order), then one thread per (structure, output) adds its chunks'
+// partials in chunk order.  The result does not depend on scheduling, and a large cell is spread
+// over many blocks instead of one.
+__global__ void __launch_bounds__(256) k_energy_partial(const double* __restrict__ Eatom,
+                                                       const long long* __restrict__ off,
+                                                       const int* __restrict__ chunk_struct,
+                                                       const int* __restrict__ chunk_first,
+                                                       int c0, int nout,
+                                                       double* __restrict__ Epart) {
   __shared__ double ws[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s = s0 + blockIdx.x;
-  const long long a0 = off[s], a1 = off[s + 1];
+  const int c = c0 + blockIdx.x;
+  const int s = chunk_struct[c];
+  const long long a0 = off[s] + (long long)(c - chunk_first[s]) * kEnergyChunk;
+  const long long a1 = min(a0 + (long long)kEnergyChunk, off[s + 1]);
   for (int t = 0; t < nout; ++t) {
     double v = 0.0;
     for (long long i = a0 + threadIdx.x; i < a1; i += 256) v += Eatom[i * nout + t];
@@ -285,9 +312,21 @@ __global__ void __launch_bounds__(256) k_energy_reduce(const double* __restrict_
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     if (lane == 0) ws[wave] = v;
     __syncthreads();
-    if (threadIdx.x == 0) Estruct[(long long)s * nout + t] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    if (threadIdx.x == 0) Epart[(long long)c * nout + t] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
     __syncthreads();
   }
+}
+
+__global__ void k_energy_final(const double* __restrict__ Epart,
+                               const int* __restrict__ chunk_first, int s0, int s1, int nout,
+                               double* __restrict__ Estruct) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long s = s0 + q / nout;
+  const int t = (int)(q % nout);
+  if (s >= s1) return;
+  double v = 0.0;
+  for (int c = chunk_first[s]; c < chunk_first[s + 1]; ++c) v += Epart[(long long)c * nout + t];
+  Estruct[s * nout + t] = v;
 }
 
 // Loss-gradient functions of loss.F90:217-281, elementwise on (prediction, target):
@@ -512,7 +551,13 @@ hipError_t launch_mlp_fwd(fnet_batch* b, const DevModel& m, int first_tile, int 
 hipError_t launch_energy_reduce(fnet_batch* b, const DevModel& m, int s0, int s1, hipStream_t s) {
   const int n = s1 - s0;
   if (n <= 0) return hipSuccess;
-  k_energy_reduce<<<n, 256, 0, s>>>(b->Eatom, b->off, s0, m.nout, b->Estruct);
+  const int c0 = b->h_chunk_first[s0], nc = b->h_chunk_first[s1] - c0;
+  if (nc > 0)
+    k_energy_partial<<<nc, 256, 0, s>>>(b->Eatom, b->off, b->chunk_struct, b->chunk_first, c0,
+                                        m.nout, b->Epart);
+  const long long q = (long long)n * m.nout;
+  k_energy_final<<<(unsigned)((q + 255) / 256), 256, 0, s>>>(b->Epart, b->chunk_first, s0, s1,
+                                                            m.nout, b->Estruct);
   return hipGetLastError();
 }
 
@@ -548,16 +593,20 @@ hipError_t launch_regularize(const DevModel& m, double* grad, double scale, doub
 
 hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad,
                            hipStream_t s) {
-  if (b->ntiles <= 0) return hipSuccess;
-  if (fp64) {
-    MlpArgs a = make_args<double>(b, m, 0);
-    a.grad = grad;
-    k_mlp_grad<double><<<b->ntiles, 256, host_lds<double>(m.shape, a.TA), s>>>(a);
-  } else {
-    MlpArgs a = make_args<float>(b, m, 0);
-    a.grad = grad;
-    k_mlp_grad<float><<<b->ntiles, 256, host_lds<float>(m.shape, a.TA), s>>>(a);
+  const long long np = m.shape.nparams * m.nspecies;
+  if (b->ntiles > 0) {
+    if (fp64) {
+      MlpArgs a = make_args<double>(b, m, 0);
+      a.grad = b->gpart;
+      k_mlp_grad<double><<<b->ntiles, 256, host_lds<double>(m.shape, a.TA), s>>>(a);
+    } else {
+      MlpArgs a = make_args<float>(b, m, 0);
+      a.grad = b->gpart;
+      k_mlp_grad<float><<<b->ntiles, 256, host_lds<float>(m.shape, a.TA), s>>>(a);
+    }
   }
+  k_grad_reduce<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(b->gpart, b->tile_sp, b->ntiles,
+                                                             m.shape.nparams, m.nspecies, grad);
   return hipGetLastError();
 }
 

@@ -14,14 +14,27 @@
 // activations go back to LDS as the next layer's rows.  The reverse sweep runs the transposed
 // products delta_l^T = W_l . delta_{l+1}^T on separately packed fragments.  Products and sums are
 // FP32 (as the generic FP32 kernel); per-atom outputs and input gradients are written in FP64.
+//
+// BF16 modes: the products whose input and output are both hidden layers (the hidden-to-hidden
+// weights, forward and reverse) run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation, 32 K per
+// instruction.  Each lane takes 8 consecutive features of its atom's row (k = 32 c + 8 g + e) and
+// the weights are packed with the same k order, so the products pair up whatever order the
+// instruction uses inside its K chunk.  Split mode (FNET_BF16) writes each operand as
+// hi + lo, hi = bf16(x) and lo = bf16(x - hi), and accumulates hi.hi + hi.lo + lo.hi: about 16
+// mantissa bits, enough for the 1e-5 eV/atom and 1e-4 eV/A gates; single mode (FNET_BF16_FAST)
+// takes hi.hi only (8 bits; its errors are reported, not gated).  The input layer (z-scored
+// features, which can be ill-conditioned) and the output layer stay FP32.
 #include <algorithm>
+#include <cstring>
 
+#include "../../include/fnet_hip.h"
 #include "fnet_internal.hpp"
 
 namespace fnet {
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -73,6 +86,10 @@ struct MfmaArgs {
   const float* wpack;
   long long pk_stride;
   long long fw[kMaxLayers], rw[kMaxLayers], pb[kMaxLayers];
+  // BF16 fragments of the hidden-to-hidden layers (hi, then lo at + bsz): forward and reverse
+  const bf8* bpack;
+  long long bk_stride;  // per species, in bf8 units
+  long long bfw[kMaxLayers], brw[kMaxLayers], bsz[kMaxLayers];
   int fpoff[kMaxLayers];  // offset (floats) of f'(z) of hidden activation layer m in the wave's store
   const float* params_f;  // [nspecies][nparams] FP32 (last-layer columns of the reverse sweep)
   const double* G;        // [N][nG]
@@ -127,7 +144,49 @@ __device__ __forceinline__ void gemm_pair(const float* __restrict__ Af, int ob0,
   }
 }
 
-template <bool GRADIN>
+// hi and lo BF16 parts of 8 FP32 values
+__device__ __forceinline__ void split8(const f4& x0, const f4& x1, bf8& h, bf8& l) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const __bf16 a = (__bf16)x0[e], b = (__bf16)x1[e];
+    h[e] = a;
+    h[e + 4] = b;
+    l[e] = (__bf16)(x0[e] - (float)a);
+    l[e + 4] = (__bf16)(x1[e] - (float)b);
+  }
+}
+
+// gemm_pair on the BF16 matrix cores: Kc32 chunks of 32; A fragments Ah (hi) and Ah + bsz (lo),
+// block stride Kc32 * 64 bf8; BF = 3 split products, 1 the hi products only.
+template <int BF>
+__device__ __forceinline__ void gemm_pair_bf(const bf8* __restrict__ Ah, long long bsz, int ob0,
+                                             int Kc32, const float* X, int ldx, int lane, int j,
+                                             int g, f4& c0, f4& c1) {
+  const bf8* A0 = Ah + (long long)ob0 * Kc32 * 64 + lane;
+  const bf8* A1 = A0 + (long long)Kc32 * 64;
+#pragma unroll 1
+  for (int c = 0; c < Kc32; ++c) {
+    const float* xr = X + j * ldx + 32 * c + 8 * g;
+    const f4 x0 = *(const f4*)xr, x1 = *(const f4*)(xr + 4);
+    bf8 xh, xl;
+    split8(x0, x1, xh, xl);
+    const bf8 a0 = A0[c * 64], a1 = A1[c * 64];
+    c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, xh, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, xh, c1, 0, 0, 0);
+    if (BF == 3) {
+      const bf8 l0 = A0[bsz + c * 64], l1 = A1[bsz + c * 64];
+      c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, xl, c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, xl, c1, 0, 0, 0);
+      c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0, xh, c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l1, xh, c1, 0, 0, 0);
+    }
+  }
+}
+
+// layer l (ww(d_l, d_{l+1})) maps a hidden layer to a hidden layer
+__device__ __forceinline__ bool hidden_hidden(int l, int nl) { return l >= 1 && l + 1 <= nl - 2; }
+
+template <bool GRADIN, int BF>
 __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
   extern __shared__ __align__(16) float smf[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -166,10 +225,15 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
     const float* Bp = WP + a.pb[l];
     const bool hidden = l < nl - 2;
     float* zs = FP + a.fpoff[l + 1];
+    const bool bf = BF && hidden_hidden(l, nl);
     for (int ob0 = 0; ob0 < Nb; ob0 += 2) {
       f4 c0 = *(const f4*)(Bp + 16 * ob0 + 4 * g);
       f4 c1 = *(const f4*)(Bp + 16 * ob0 + 16 + 4 * g);
-      gemm_pair(Wf, ob0, Kc, X, ldx, lane, j, g, c0, c1);
+      if (bf)
+        gemm_pair_bf<BF>(a.bpack + (long long)sp * a.bk_stride + a.bfw[l], a.bsz[l], ob0, Kc >> 1,
+                         X, ldx, lane, j, g, c0, c1);
+      else
+        gemm_pair(Wf, ob0, Kc, X, ldx, lane, j, g, c0, c1);
       if (hidden) {
         *(f4*)(zs + (ob0 * 64 + lane) * 4) = c0;
         *(f4*)(zs + ((ob0 + 1) * 64 + lane) * 4) = c1;
@@ -228,9 +292,14 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
     const int Mb = r32(a.sh.dims[m - 1]) >> 4, Kc = r32(a.sh.dims[m]) >> 4;
     const float* Rf = WP + a.rw[m - 1];
     float* sl = FP + a.fpoff[m - 1];
+    const bool bf = BF && hidden_hidden(m - 1, nl);
     for (int ib0 = 0; ib0 < Mb; ib0 += 2) {
       f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
-      gemm_pair(Rf, ib0, Kc, X, ldx, lane, j, g, c0, c1);
+      if (bf)
+        gemm_pair_bf<BF>(a.bpack + (long long)sp * a.bk_stride + a.brw[m - 1], a.bsz[m - 1], ib0,
+                         Kc >> 1, X, ldx, lane, j, g, c0, c1);
+      else
+        gemm_pair(Rf, ib0, Kc, X, ldx, lane, j, g, c0, c1);
       if (m - 1 >= 1) {
         f4* s0 = (f4*)(sl + (ib0 * 64 + lane) * 4);
         f4* s1 = (f4*)(sl + ((ib0 + 1) * 64 + lane) * 4);
@@ -324,6 +393,78 @@ void pack_mfma_weights(const NetShape& sh, int nspecies, const double* params,
   }
 }
 
+namespace {
+// float -> bf16 bits, round to nearest even (finite values)
+uint16_t to_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float from_bf16(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+}  // namespace
+
+// BF16 fragments (hi, then lo = bf16(w - hi)) of the hidden-to-hidden layers for
+// v_mfma_f32_16x16x32_bf16.  Forward fragment of layer l: element
+// ((ob * Kc32 + c) * 64 + lane) * 8 + e, lane = 16 g + i, holds ww_l(k = 32 c + 8 g + e,
+// o = 16 ob + i); reverse: ((ib * Oc32 + c) * 64 + lane) * 8 + e holds
+// ww_l(k = 16 ib + i, o = 32 c + 8 g + e).  Offsets and sizes are in units of 8 values.
+void pack_bf16_weights(const NetShape& sh, int nspecies, const double* params,
+                       std::vector<uint16_t>& out, long long& stride, long long* bfw,
+                       long long* brw, long long* bsz) {
+  long long o = 0;
+  for (int l = 0; l < kMaxLayers; ++l) bfw[l] = brw[l] = bsz[l] = 0;
+  for (int l = 1; l + 2 < sh.nl; ++l) {
+    const long long sz = (long long)r32(sh.dims[l]) * r32(sh.dims[l + 1]) / 8;
+    bsz[l] = sz;
+    bfw[l] = o;
+    o += 2 * sz;
+    brw[l] = o;
+    o += 2 * sz;
+  }
+  stride = o;
+  out.assign((size_t)(stride * 8 * nspecies), 0);
+  for (int sp = 0; sp < nspecies; ++sp) {
+    const double* P = params + (long long)sp * sh.nparams;
+    uint16_t* D = out.data() + (long long)sp * stride * 8;
+    for (int l = 1; l + 2 < sh.nl; ++l) {
+      const int din = sh.dims[l], dout = sh.dims[l + 1];
+      const int Kc32 = r32(din) / 32, Oc32 = r32(dout) / 32;
+      const int Kb = r32(din) / 16, Ob = r32(dout) / 16;
+      const long long sz8 = bsz[l] * 8;
+      auto put = [&](long long base, long long idx, float w) {
+        const uint16_t h = to_bf16(w);
+        D[base + idx] = h;
+        D[base + sz8 + idx] = to_bf16(w - from_bf16(h));
+      };
+      auto ww = [&](int k, int oo) -> float {
+        return (k < din && oo < dout) ? (float)P[sh.woff[l] + (long long)oo * din + k] : 0.0f;
+      };
+      for (int ob = 0; ob < Ob; ++ob)
+        for (int c = 0; c < Kc32; ++c)
+          for (int lane = 0; lane < 64; ++lane)
+            for (int e = 0; e < 8; ++e) {
+              const int g = lane / 16, i = lane % 16;
+              put(bfw[l] * 8, (((long long)ob * Kc32 + c) * 64 + lane) * 8 + e,
+                  ww(32 * c + 8 * g + e, 16 * ob + i));
+            }
+      for (int ib = 0; ib < Kb; ++ib)
+        for (int c = 0; c < Oc32; ++c)
+          for (int lane = 0; lane < 64; ++lane)
+            for (int e = 0; e < 8; ++e) {
+              const int g = lane / 16, i = lane % 16;
+              put(brw[l] * 8, (((long long)ib * Oc32 + c) * 64 + lane) * 8 + e,
+                  ww(16 * ib + i, 32 * c + 8 * g + e));
+            }
+    }
+  }
+}
+
 hipError_t launch_mlp_mfma(fnet_batch* b, const DevModel& m, int first_tile, int ntiles,
                            int want_grad_in, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
@@ -353,10 +494,27 @@ hipError_t launch_mlp_mfma(fnet_batch* b, const DevModel& m, int first_tile, int
   a.first_tile = first_tile;
   a.Eatom = b->Eatom;
   a.cG = (double*)b->cG;
+  a.bpack = (const bf8*)m.bpack;
+  a.bk_stride = m.bk_stride;
+  for (int l = 0; l < kMaxLayers; ++l) {
+    a.bfw[l] = m.bk_fw[l];
+    a.brw[l] = m.bk_rw[l];
+    a.bsz[l] = m.bk_sz[l];
+  }
+  const int bf = (b->precision == FNET_BF16 && m.bpack) ? 3
+               : (b->precision == FNET_BF16_FAST && m.bpack) ? 1 : 0;
   const int waves = b->mtile_waves;
   const size_t lds = (size_t)waves * a.wave_floats * sizeof(float);
-  if (want_grad_in) k_mlp_mfma<true><<<ntiles, 64 * waves, lds, s>>>(a);
-  else k_mlp_mfma<false><<<ntiles, 64 * waves, lds, s>>>(a);
+  const dim3 grid(ntiles), block(64 * waves);
+  if (want_grad_in) {
+    if (bf == 3) k_mlp_mfma<true, 3><<<grid, block, lds, s>>>(a);
+    else if (bf == 1) k_mlp_mfma<true, 1><<<grid, block, lds, s>>>(a);
+    else k_mlp_mfma<true, 0><<<grid, block, lds, s>>>(a);
+  } else {
+    if (bf == 3) k_mlp_mfma<false, 3><<<grid, block, lds, s>>>(a);
+    else if (bf == 1) k_mlp_mfma<false, 1><<<grid, block, lds, s>>>(a);
+    else k_mlp_mfma<false, 0><<<grid, block, lds, s>>>(a);
+  }
   return hipGetLastError();
 }
 

@@ -11,6 +11,8 @@
 // them: |row_a(inv L) . dr| <= rc |row_a(inv L)| bounds every fractional offset of a neighbour,
 // so an image n with n_a outside [s_a(i) - smax_a - w_a, s_a(i) - smin_a + w_a] holds none.
 // The exact test still decides every candidate that is looked at.
+#include <algorithm>
+
 #include "fnet_internal.hpp"
 
 namespace fnet {
@@ -147,6 +149,7 @@ __global__ void __launch_bounds__(256)
             int* __restrict__ cnt, const int* __restrict__ rowptr,
             int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd,
             const int4* __restrict__ cellinfo, int cap) {
+  // nimg receives each entry's key (image + 1) n + (j - i) mod n
 #pragma clang fp contract(off)
   constexpr bool WRITE = MODE != kCount;
   const int lane = threadIdx.x & 63;
@@ -163,13 +166,15 @@ __global__ void __launch_bounds__(256)
   const long long base = MODE == kFill ? rowptr[i] : (MODE == kPadded ? i * cap : 0);
   int count = 0;
 
-  auto emit = [&](bool hit, int j, int img, double nd0, double nd1, double nd2, double d2) {
+  // key of an entry: (image + 1) n + (j - i) mod n, which increases along the row (the image
+  // index is key / n - 1); the force gather's mirror search reads it
+  auto emit = [&](bool hit, int j, int key, double nd0, double nd1, double nd2, double d2) {
     const unsigned long long mask = __ballot(hit);
     const int q = count + __popcll(mask & lanemask_lt(lane));
     if (WRITE && hit && (MODE != kPadded || q < cap)) {
       const long long pos = base + q;
       nj[pos] = (int)(o + j);
-      nimg[pos] = img;
+      nimg[pos] = key;
       V v;
       v.x = nd0;
       v.y = nd1;
@@ -188,7 +193,7 @@ __global__ void __launch_bounds__(256)
     double nd0 = 0, nd1 = 0, nd2 = 0, d2 = 0;
     bool hit = false;
     if (t < n) hit = near_test(ci0, ci1, ci2, X, j, 0.0, 0.0, 0.0, rc2, nd0, nd1, nd2, d2);
-    emit(hit, j, -1, nd0, nd1, nd2, d2);
+    emit(hit, j, t, nd0, nd1, nd2, d2);
   }
   const StructInfo& si = sinfo[s];
   if (si.periodic) {
@@ -252,7 +257,7 @@ __global__ void __launch_bounds__(256)
           img = (int)c.w;
           hit = near_test(ci0, ci1, ci2, X, j, c.x, c.y, c.z, rc2, nd0, nd1, nd2, d2);
         }
-        emit(hit, j, img, nd0, nd1, nd2, d2);
+        emit(hit, j, (img + 1) * n + t, nd0, nd1, nd2, d2);
         slot += qs;
         t += rs;
         if (t >= n) {
@@ -356,7 +361,30 @@ __global__ void k_scan_add(int* __restrict__ out, long long n, const int* __rest
   if (g < n) out[g] += totals[blockIdx.x];
 }
 
+// Longest row of the list: block maxima combined with atomicMax into *out (zeroed before).
+__global__ void __launch_bounds__(256) k_row_max(const int* __restrict__ cnt, long long n,
+                                                 int* __restrict__ out) {
+  __shared__ int wm[4];
+  int m = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    m = max(m, cnt[i]);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, max(max(wm[0], wm[1]), max(wm[2], wm[3])));
+}
+
 }  // namespace
+
+hipError_t launch_row_max(const int* cnt, long long n, int* out, hipStream_t s) {
+  FNET_HIP_TRY(launch_zero_int(out, 1, s));
+  if (n <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>(1024, (n + 255) / 256);
+  k_row_max<<<(unsigned)blocks, 256, 0, s>>>(cnt, n, out);
+  return hipGetLastError();
+}
 
 // scratch of launch_scan: block totals (nb + 1) and block maxima (nb)
 long long scan_tmp_size(long long n) { return 2 * ((n + kScanBlock - 1) / kScanBlock) + 1; }

@@ -112,7 +112,7 @@ program fnet_fortran_check
     call gpu%trainGradients(targets, atomicWeights, dpWeights, grad, loss)
     call gpu%steepestDescent(lr, maxDisp, tol, params, tConverged)
     write(fo, '(a)') 'LOSS 1 1'
-    write(fo, '(es25.17)') loss
+    write(fo, '(es26.17e3)') loss
     call writeBlock(fo, 'GRAD', grad)
     call writeBlock(fo, 'PARAMS', params)
   end if
@@ -130,7 +130,7 @@ contains
     write(fo, '(a, 1x, i0, 1x, i0)') name, size(array, dim=1), size(array, dim=2)
     do i2 = 1, size(array, dim=2)
       do i1 = 1, size(array, dim=1)
-        write(fo, '(es25.17)') array(i1, i2)
+        write(fo, '(es26.17e3)') array(i1, i2)
       end do
     end do
   end subroutine writeBlock

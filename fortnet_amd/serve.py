@@ -4,10 +4,13 @@ The reference's i-PI driver (``prg_fnet/fortnet.F90:126-169``) keeps the network
 per request, receives one structure, recomputes its ACSF values and derivatives, runs the
 networks and returns energy and forces -- one request at a time.  Here requests that arrive while
 others are on the GPU are queued and admitted into the next device batch as soon as a batch slot
-frees up, up to an atom budget derived from the HBM footprint per atom.  Two slots (each a
-context with its own HIP stream and a resident batch whose buffers are reused from batch to
-batch) let one batch be assembled and launched while the other computes, so their work overlaps
-on the device.  ``serial_loop`` is the one-at-a-time loop for comparison.
+frees up, up to an atom budget set by the HBM footprint: the device bytes per atom are measured
+once (free device memory before and after a calibration batch is built and run), and each
+admission takes as many queued atoms as the slot's resident buffers plus its share of the
+currently free device memory hold.  Two slots (each a context with its own HIP stream and a
+resident batch whose buffers are reused from batch to batch) let one batch be assembled and
+launched while the other computes, so their work overlaps on the device.  ``serial_loop`` is the
+one-at-a-time loop for comparison.
 
 Latency of a request = time its results are on the host - its arrival time.
 """
@@ -20,10 +23,11 @@ from .core import Context
 from ._lib import FNET_FP32
 
 
-# bytes per atom resident for one evaluation: neighbour entries (~n_nb x (16 + 4 + 4) bytes FP32),
-# coordinates, ACSF values and their gradient (2 x nG FP64), energies and forces
+# first guess of the device bytes per atom before a calibration: neighbour entries in padded rows
+# (index, key, displacement, mirror, orphan slot, FP64 gradient vector: ~72 bytes each, ~160 per
+# atom), coordinates, ACSF values and their gradient (2 x nG FP64), energies and forces
 def bytes_per_atom(n_features, n_neighbours=160):
-    return n_neighbours * 24 + 64 + 16 * n_features + 64
+    return n_neighbours * 72 + 64 + 16 * n_features + 64
 
 
 class Request:
@@ -59,9 +63,11 @@ class _Slot:
     def busy(self):
         return bool(self.reqs) and self.batch.query()
 
-    def finish(self, t):
+    def finish(self, now):
+        """Downloads the results; ``now()`` is read once they are on the host."""
         E, _ = self.batch.energy()
         F = self.batch.forces()
+        t = now()
         off = self.batch.offsets
         for s, r in enumerate(self.reqs):
             r.energy = E[s].copy()
@@ -78,26 +84,53 @@ class _Slot:
 
 
 class BatchingServer:
-    """Continuous batching over ``n_slots`` device slots."""
+    """Continuous batching over ``n_slots`` device slots, admission by free device memory."""
 
-    def __init__(self, netstat, precision=FNET_FP32, hbm_budget_bytes=8 << 30,
+    def __init__(self, netstat, precision=FNET_FP32, hbm_fraction=0.5,
                  max_atoms_per_batch=1 << 18, n_slots=2, device=0):
         self.slots = [_Slot(netstat, precision, device) for _ in range(n_slots)]
-        per_atom = bytes_per_atom(len(netstat.g))
-        self.max_atoms = int(min(max_atoms_per_batch, hbm_budget_bytes // (per_atom * n_slots)))
+        self.per_atom = float(bytes_per_atom(len(netstat.g)))
+        self.fraction = hbm_fraction
+        self.max_atoms = int(max_atoms_per_batch)
+        self.held = [0] * n_slots   # atoms the slot's resident buffers were sized for
         self.batch_sizes = []
+        self.budgets = []
 
     def close(self):
         for s in self.slots:
             s.close()
 
-    def _admit(self, queue):
-        """Requests for the next batch: FIFO while the atom budget allows (at least one)."""
+    def calibrate(self, geos):
+        """Measures the device bytes per atom: free device memory before and after the first
+        slot's batch is built and run on ``geos`` (a few representative structures)."""
+        slot = self.slots[0]
+        free0, _ = slot.ctx.device_memory()
+        slot.launch([Request(-1, g, 0.0) for g in geos])
+        slot.batch.sync()
+        slot.finish(time.perf_counter)
+        free1, _ = slot.ctx.device_memory()
+        atoms = sum(g.n for g in geos)
+        if atoms > 0 and free0 > free1:
+            self.per_atom = max(self.per_atom * 0.25, (free0 - free1) / atoms)
+        self.held[0] = atoms
+        return self.per_atom
+
+    def _budget(self, k):
+        """Atoms slot k may take now: its resident buffers plus its share of the free memory."""
+        free, _ = self.slots[k].ctx.device_memory()
+        share = free * self.fraction / len(self.slots)
+        return int(min(self.max_atoms, self.held[k] + share / self.per_atom))
+
+    def _admit(self, queue, k):
+        """Requests for slot k's next batch: FIFO while the atom budget allows (at least one)."""
+        budget = self._budget(k)
+        self.budgets.append(budget)
         take, atoms = [], 0
-        while queue and (not take or atoms + queue[0].geo.n <= self.max_atoms):
+        while queue and (not take or atoms + queue[0].geo.n <= budget):
             r = queue.popleft()
             take.append(r)
             atoms += r.geo.n
+        self.held[k] = max(self.held[k], atoms)
         return take
 
     def run(self, requests, t0=None):
@@ -117,11 +150,11 @@ class BatchingServer:
             progressed = False
             for s in self.slots:
                 if s.reqs and not s.batch.query():
-                    ndone += len(s.finish(clock() - t0))
+                    ndone += len(s.finish(lambda: clock() - t0))
                     progressed = True
-            for s in self.slots:
+            for k, s in enumerate(self.slots):
                 if not s.reqs and queue:
-                    take = self._admit(queue)
+                    take = self._admit(queue, k)
                     self.batch_sizes.append(len(take))
                     s.launch(take)
                     progressed = True
@@ -148,7 +181,7 @@ def serial_loop(netstat, requests, precision=FNET_FP32, device=0, t0=None):
                 time.sleep(wait)
             slot.launch([r])
             slot.batch.sync()
-            slot.finish(clock() - t0)
+            slot.finish(lambda: clock() - t0)
     finally:
         slot.close()
     return reqs

@@ -42,20 +42,25 @@ def train(ctx, netstat, geos, g_targets=None, a_targets=None, atomic_weights=Non
           dp_weights=None, optimizer="sd", n_iter=1, learning_rate=0.01, tol=1e-20,
           min_disp=1e-6, max_disp=1e4, memory=1000, line_search=True, loss="mse",
           regularization=None, reg_strength=0.0, reg_alpha=0.0, shuffle=False, rng=None,
-          precision=FNET_FP64):
+          precision=FNET_FP64, device_optimizer=False):
     """Runs ``n_iter`` training iterations from the parameters of ``netstat`` (uploaded as the
-    context's model) and returns a :class:`TrainResult`; the context keeps the final model."""
+    context's model) and returns a :class:`TrainResult`; the context keeps the final model.
+    ``device_optimizer``: the FP64 optimizer step runs on the GPU (``fnet_optimizer_step``)
+    instead of the host state machines."""
     ctx.upload_model(netstat)
     ctx.set_loss(loss)
     ctx.set_regularization(regularization, reg_strength, reg_alpha)
     b = ctx.batch(geos, precision)
     nsp, npar = netstat.params.shape
     opts = []
-    for sp in range(nsp):
-        o = optim.make(optimizer, npar, tol, min_disp, max_disp, learning_rate, memory,
-                       line_search)
-        o.reset(netstat.params[sp])
-        opts.append(o)
+    if device_optimizer:
+        ctx.optimizer_init(optimizer, tol, min_disp, max_disp, learning_rate, memory, line_search)
+    else:
+        for sp in range(nsp):
+            o = optim.make(optimizer, npar, tol, min_disp, max_disp, learning_rate, memory,
+                           line_search)
+            o.reset(netstat.params[sp])
+            opts.append(o)
     order = list(range(len(geos)))
     if shuffle and rng is not None:
         knuth_shuffle(rng, order)
@@ -69,12 +74,16 @@ def train(ctx, netstat, geos, g_targets=None, a_targets=None, atomic_weights=Non
                  for l in range(len(netstat.dims) - 1)) + int(netstat.dims[-1]))
     for it in range(1, n_iter + 1):
         norms.append(float(np.sqrt(np.sum(grad ** 2))))
-        new = np.empty_like(params)
-        for sp in range(nsp):
-            fx = loss_val + _regu_loss(params[sp, :nw], regularization, reg_strength, reg_alpha)
-            new[sp], converged = opts[sp].next(fx, grad[sp])
-        params = new
-        ctx.upload_params(params)
+        if device_optimizer:
+            params, converged = ctx.optimizer_step()
+        else:
+            new = np.empty_like(params)
+            for sp in range(nsp):
+                fx = loss_val + _regu_loss(params[sp, :nw], regularization, reg_strength,
+                                           reg_alpha)
+                new[sp], converged = opts[sp].next(fx, grad[sp])
+            params = new
+            ctx.upload_params(params)
         if shuffle and rng is not None:
             knuth_shuffle(rng, order)
         grad, _ = b.train_gradients(g_targets, a_targets, atomic_weights, dp_weights)

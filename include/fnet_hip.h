@@ -33,6 +33,11 @@ enum fnet_precision {
   FNET_FP32 = 1, /* ACSF, layers and forces FP32; energy/loss/gradient reductions FP64 */
   FNET_FP32_NET64 = 2, /* ACSF and forces FP32, element networks FP64 (ill-conditioned
                           z-scores: the FP32 input gradient limits the force error) */
+  FNET_BF16 = 3,       /* as FNET_FP32, with the hidden-to-hidden layer products on BF16 MFMA
+                          with FP32 accumulation, each operand split into bf16 hi + lo parts
+                          (hi.hi + hi.lo + lo.hi: within the 1e-5 eV/atom, 1e-4 eV/A gates) */
+  FNET_BF16_FAST = 4,  /* as FNET_BF16 with the hi.hi products only (8-bit mantissas; its
+                          errors are reported, not within the gates) */
 };
 
 /* Launch shape. */
@@ -140,8 +145,26 @@ int fnet_energy(fnet_ctx* ctx, fnet_batch* batch, double* E_struct, double* E_at
 int fnet_forces(fnet_ctx* ctx, fnet_batch* batch, double* F);
 
 /* One full energy+force evaluation on the device (neighbour list, ACSF, networks, adjoint),
- * leaving results on the device; the serving / benchmark step. */
+ * leaving results on the device; the serving / benchmark step (the predict-with-forces sequence
+ * of prg_fnet/fortnet.F90:176-205 for a whole batch).  After the first step of a batch it
+ * enqueues every stage without a host read-back: the neighbour search runs with the row capacity
+ * of the previous list, and the longest row is checked when results are next read (fnet_energy,
+ * fnet_forces, fnet_batch_sync, ...), which recomputes the step if a row outgrew the capacity.
+ * It returns before the step has run. */
 int fnet_run(fnet_ctx* ctx, fnet_batch* batch);
+
+/* 1: fnet_run replays a HIP graph of its steady-state step, captured on first use and again
+ * whenever the batch's buffers, capacities or the model change; 0 (default): eager launches.
+ * (No reference counterpart.) */
+int fnet_set_graph(fnet_ctx* ctx, int enable);
+
+/* Waits for the batch's last step and validates it (see fnet_run). */
+int fnet_batch_sync(fnet_batch* batch);
+
+/* Device time of the whole last fnet_run / fnet_forces step in ms (HIP events on the context
+ * stream); *graph = 1 if the batch's last fnet_run replayed its graph. */
+int fnet_step_time(fnet_ctx* ctx, double* ms);
+int fnet_batch_step_info(const fnet_batch* batch, int* graph);
 
 /* Training gradient of the whole batch: sysTrain + updateGradients + the normalisation of
  * update (bpnn.F90:394-481,610-704,751-767) with the loss gradient set by fnet_set_loss
@@ -179,6 +202,24 @@ int fnet_set_regularization(fnet_ctx* ctx, double strength, double alpha);
 int fnet_optimizer_step_sd(fnet_ctx* ctx, double lr, double max_disp, double tol,
                            double* params_out, int* converged);
 
+/* Optimizers of the training loop on the device, FP64, one per element network: the state
+ * machines of TOptimizer_next (optimizers.F90:184-221) -- steepest descent (steepdesc.F90:125-204),
+ * conjugate gradients (conjgrad.F90), L-BFGS with or without the line minimiser (lbfgs.F90:283-488,
+ * linemin.F90) and FIRE (fire.F90) -- with the Training block's settings
+ * (initprogram.F90:1130-1185).  fnet_optimizer_init starts them at the resident parameters. */
+enum fnet_optimizer { FNET_OPT_SD = 0, FNET_OPT_CG = 1, FNET_OPT_LBFGS = 2, FNET_OPT_FIRE = 3 };
+int fnet_optimizer_init(fnet_ctx* ctx, int kind, double tol, double min_disp, double max_disp,
+                        double learning_rate, int memory, int line_search);
+/* One iteration of every network's optimizer with the gradient (and loss) of the last
+ * fnet_train_gradients call: the resident parameters move to the next point to evaluate (as
+ * TBpnn_update, bpnn.F90:708-778).  params_out [n_species][n_params] (may be NULL) receives them;
+ * converged receives the flag of the last network, as the reference's training loop reads it. */
+int fnet_optimizer_step(fnet_ctx* ctx, int kind, double* params_out, int* converged);
+
+/* Free and total device memory in bytes (hipMemGetInfo): the serving loop admits requests by it.
+ * (No reference counterpart.) */
+int fnet_device_memory(fnet_ctx* ctx, long long* free_bytes, long long* total_bytes);
+
 /* Blocks until the context's stream has drained. */
 int fnet_sync(fnet_ctx* ctx);
 /* Non-blocking: 0 if all work queued on the context has finished, 1 if some is still running,
@@ -186,8 +227,9 @@ int fnet_sync(fnet_ctx* ctx);
 int fnet_query(fnet_ctx* ctx);
 
 /* Timing of the last fnet_run / fnet_forces call, per stage, in milliseconds (HIP events on
- * the context stream): [0] neighbour list, [1] ACSF forward, [2] networks, [3] adjoint forces,
- * [4] energy reduction.  n = number of entries written (<= 8). */
+ * the context stream): [0] neighbour list (with the mirror entries), [1] ACSF forward,
+ * [2] networks, [3] adjoint forces (with the force gather), [4] energy reduction.
+ * n = number of entries written (<= 8).  A step replayed as a graph has only its total, in [0]. */
 int fnet_stage_times(fnet_ctx* ctx, double* ms, int n);
 
 #ifdef __cplusplus

@@ -32,6 +32,10 @@
 #define ORC_REFSHAPE 1      /* rebuild the neighbour list per (atom, function) like acsf.F90:829-834 */
 #define ORC_COMPAT_SCATTER 2 /* last-write-wins dense scatter of acsf.F90:918-933 */
 #define ORC_COMPAT_SELFIMG 4 /* keep the centre's own images in species-resolved lists, acsf.F90:754-760 */
+/* A planted mistake for the tests (never a reference behaviour): cos(theta) of the G4/G5 values
+ * as dot * (1/r_j) * (1/r_k) without the 1e-13 regulariser of acsf.F90:1173-1174, and exactly 1
+ * for j == k.  The tests show that the reference's element-wise tolerance detects it. */
+#define ORC_UNREG_COS 8
 
 typedef struct {
   int nG;
@@ -285,7 +289,7 @@ static inline double theta(const double* c, const double* n1, const double* n2, 
 /* G-function values: g1 :1287-1310, g2 :1314-1343, g3 :1347-1373, g4 :1377-1436, g5 :1440-1492 */
 
 static double gvalue(const orc_gfuncs* gf, int a, const double* ci, double qi, const flist* l1,
-                     const flist* l2) {
+                     const flist* l2, int flags) {
   const int ty = gf->type[a];
   const double rc = gf->rcut[a];
   double s = 0.0;
@@ -315,7 +319,15 @@ static double gvalue(const orc_gfuncs* gf, int a, const double* ci, double qi, c
     for (long k = 0; k < l2->n; ++k) {
       const double* nk = l2->xyz + 3 * k;
       const double rk = l2->r[k];
-      double base = pow(1.0 + lam * cos(theta(ci, nj, nk, rj, rk)), xi);
+      double cth;
+      if (flags & ORC_UNREG_COS) {
+        const double u[3] = {nj[0] - ci[0], nj[1] - ci[1], nj[2] - ci[2]};
+        const double v[3] = {nk[0] - ci[0], nk[1] - ci[1], nk[2] - ci[2]};
+        cth = (l1 == l2 && j == k) ? 1.0 : dot3(u, v) * (1.0 / rj) * (1.0 / rk);
+      } else {
+        cth = cos(theta(ci, nj, nk, rj, rk));
+      }
+      double base = pow(1.0 + lam * cth, xi);
       if (ty == 4) {
         double djk[3] = {nj[0] - nk[0], nj[1] - nk[1], nj[2] - nk[2]};
         double distjk = norm3(djk);
@@ -517,7 +529,7 @@ static int run_structure(int what, int n, const double* xyz, const double* lat, 
         pl2 = &l2;
       }
       if (what == 1) {
-        G[(long)i * nG + a] = gvalue(gf, a, ci, qi, &l1, pl2);
+        G[(long)i * nG + a] = gvalue(gf, a, ci, qi, &l1, pl2, flags);
         continue;
       }
       const int same = (gf->z1[a] == gf->z2[a]);

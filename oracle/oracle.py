@@ -30,6 +30,7 @@ SRC = os.path.join(HERE, "fnet_oracle.c")
 REFSHAPE = 1
 COMPAT_SCATTER = 2
 COMPAT_SELFIMG = 4
+UNREG_COS = 8  # planted mistake for the tests: cos(theta) without the 1e-13 regulariser
 
 ACTIVATIONS = ["tanh", "sigmoid", "relu", "lrelu", "softplus", "bent", "atan", "gaussian",
                "heaviside", "linear"]

@@ -2,10 +2,11 @@
 
 Mirrors ``test/prog/fortnet/bin/testwithworkdir.py``: a netstat + dataset go in, the predicted
 ``rawpredictions`` / ``globalpredictions`` / ``forces`` of fnetout must come out.  The FP64 path
-is held to the reference's own tolerance (ATOL 1e-10, RTOL 1e-9, testwithworkdir.py:24-25);
-the FP32 path to a relative 1e-3 of the largest value of the quantity: several fixture
-z-scores have sigma ~5e-4 for features of size ~1, so the FP32 rounding of G is amplified
-~2000-fold before it reaches the networks.
+is held to the reference's own element-wise check, ``np.allclose(got, want, atol=1e-10,
+rtol=1e-9)`` (testwithworkdir.py:24-25,700-714); the FP32 path to a relative 1e-3 of the
+largest value of the quantity: several fixture z-scores have sigma ~5e-4 (down to 9e-8) for
+features of size ~1, so the FP32 rounding of G is amplified ~2000-fold (and more) before it
+reaches the networks.
 """
 import numpy as np
 import pytest
@@ -32,11 +33,8 @@ def ctx():
 
 def _check(got, want, precision, what):
     if precision == fa.FNET_FP64:
-        # the reference's tolerance, with RTOL taken relative to the largest value: the device
-        # sums in a different order than the reference, and near-zero force components inherit
-        # the rounding of the large terms that cancel in them
-        tol = ATOL + RTOL * max(1.0, float(np.abs(want).max()))
-        assert np.abs(got - want).max() <= tol, \
+        # the reference's element-wise check (testwithworkdir.py:700-714)
+        assert np.allclose(got, want, atol=ATOL, rtol=RTOL), \
             "%s: max abs err %.3e" % (what, np.abs(got - want).max())
     else:
         scale = max(1.0, float(np.abs(want).max()))

@@ -8,9 +8,11 @@ regularisation.  Reproducing the stored parameters pins, in one chain, the gener
 initialisation (random.F90, ranlux.F90, network.F90:95-122), the z-score statistics
 (acsf.F90:445-490), the activation of every case's network, the loss gradients (loss.F90), the
 regularisation (nestedtypes.F90:336-370), the gradient normalisation (bpnn.F90:610-766) and each
-optimizer's first step.  Parameters must agree to the reference's tolerance, ATOL 1e-10 plus
-RTOL 1e-9 of the largest parameter (testwithworkdir.py:24-25), in FP64.
+optimizer's first step.  Parameters must agree element-wise to the reference's tolerance,
+``np.allclose(atol=1e-10, rtol=1e-9)`` (testwithworkdir.py:24-25,301-525), in FP64.
 The restart-style cases of ``input/*`` (an input netstat, one SD step) are checked the same way.
+Every case runs twice: with the optimizer step on the host state machines and with the FP64
+device optimizers (``fnet_optimizer_step``).
 """
 import numpy as np
 import pytest
@@ -58,12 +60,12 @@ def _targets(ds, ns):
 
 
 def _close(got, want, what):
-    tol = ATOL + RTOL * max(1.0, float(np.abs(want).max()))
+    # element-wise, as the reference's netstat comparison (testwithworkdir.py:301-525)
     err = float(np.abs(got - want).max())
-    assert err <= tol, "%s: max abs err %.3e (tol %.3e)" % (what, err, tol)
+    assert np.allclose(got, want, atol=ATOL, rtol=RTOL), "%s: max abs err %.3e" % (what, err)
 
 
-def _run(ctx, c, ds, ns, rng):
+def _run(ctx, c, ds, ns, rng, device_optimizer=False):
     gt, at = _targets(ds, ns)
     regu = str(c["regu_type"]) or None
     return train(ctx, ns, dataset.geometries(ds), gt, at, ds["atomicweights"],
@@ -74,13 +76,19 @@ def _run(ctx, c, ds, ns, rng):
                  memory=int(c["memory"]), line_search=bool(int(c["linemin"])),
                  loss=str(c["loss"]), regularization=regu,
                  reg_strength=float(c["regu_strength"]), reg_alpha=float(c["regu_alpha"]),
-                 shuffle=str(c["shuffle"]) == "yes", rng=rng)
+                 shuffle=str(c["shuffle"]) == "yes", rng=rng, device_optimizer=device_optimizer)
 
 
+# the optimizer step on the host state machines or on the device (fnet_optimizer_step)
+OPT_WHERE = [False, True]
+OPT_IDS = ["host_opt", "device_opt"]
+
+
+@pytest.mark.parametrize("device_opt", OPT_WHERE, ids=OPT_IDS)
 @pytest.mark.parametrize("name", SCRATCH)
-def test_training_from_scratch(ctx, name):
+def test_training_from_scratch(ctx, name, device_opt):
     c, ds = load_case(name)
-    print("case", name, flush=True)
+    print("case", name, "device optimizer", device_opt, flush=True)
     ref = Netstat.from_arrays(c, "after_")
     geos = dataset.geometries(ds)
     zmean = zstd = None
@@ -97,14 +105,15 @@ def test_training_from_scratch(ctx, name):
         _close(zstd, c["after_zstd"], "z-score deviations")
     ns, gen = scratch_netstat(ref.g, ref.species_z, ref.dims, ref.activation, int(c["seed"]),
                               zmean, zstd, ref.ext_in, ref.n_global)
-    res = _run(ctx, c, ds, ns, gen)
+    res = _run(ctx, c, ds, ns, gen, device_opt)
     _close(res.params, c["after_params"], "parameters after one iteration")
 
 
+@pytest.mark.parametrize("device_opt", OPT_WHERE, ids=OPT_IDS)
 @pytest.mark.parametrize("name", RESTART)
-def test_training_restart(ctx, name):
+def test_training_restart(ctx, name, device_opt):
     c, ds = load_case(name)
-    print("case", name, flush=True)
+    print("case", name, "device optimizer", device_opt, flush=True)
     ns = Netstat.from_arrays(c, "in_")
-    res = _run(ctx, c, ds, ns, None)
+    res = _run(ctx, c, ds, ns, None, device_opt)
     _close(res.params, c["after_params"], "parameters after one iteration")

@@ -208,4 +208,7 @@ def test_water_box_steps_match_direct_evaluation(tmp_path):
         bb.close()
         assert abs(e - Es[0, 0]) <= 1e-9 * max(1.0, abs(Es[0, 0]))
         assert np.abs(f - F).max() <= 1e-9 * max(1.0, np.abs(F).max())
+        # and the FP64 oracle, within the FP32 accuracy gates
+        from test_gpu_serving import check_against_oracle
+        check_against_oracle(ns, geo, e, f, fa.FNET_FP32)
     ctx.close()

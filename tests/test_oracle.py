@@ -67,6 +67,23 @@ def test_fixture_dense_forces(name):
         assert _close(F, c["out_forces"])
 
 
+def test_cos_regulariser_decides_force_parity():
+    """The reference evaluates cos(theta) of the G4/G5 values as cos(acos(d / (r_j r_k + 1e-13)))
+    (acsf.F90:1159-1176).  Dropping the 1e-13 (and taking j == k as exactly 1) moves each cosine
+    by only ~1e-15, always in one direction; this fixture's z-scores have sigma down to 9e-8, so
+    the shift reaches the forces at ~1e-9 and fails the reference's element-wise tolerance.  The
+    planted mistake sits in the oracle (flag UNREG_COS); the device FP64 path keeps the
+    regulariser for this reason."""
+    name = "analysis__derivatives__analytical__multiTargets__speciesResolved__raw"
+    c, ds = load_case(name)
+    _, _, F = O.predict_case(c, ds, forces=True)
+    assert _close(F, c["out_forces"])
+    _, _, Fbad = O.predict_case(c, ds, flags=O.UNREG_COS, forces=True)
+    assert not _close(Fbad, c["out_forces"])
+    excess = np.abs(Fbad - c["out_forces"]) / (ATOL + RTOL * np.abs(c["out_forces"]))
+    assert excess.max() > 2.0
+
+
 SD = [n for n in case_names("restart") if n.endswith("steepestdesc") or "acsffeatures" in n]
 
 

@@ -61,7 +61,7 @@ def test_oracle_sd_step_from_scratch(name):
         zm = (wa[:, None] * G).sum(axis=0) / wa.sum()
         zs = np.sqrt((wa[:, None] * (G - zm) ** 2).sum(axis=0) / wa.sum())
         for got, want in ((zm, c["after_zmean"]), (zs, c["after_zstd"])):
-            assert np.abs(got - want).max() <= ATOL + RTOL * max(1.0, np.abs(want).max())
+            assert np.allclose(got, want, atol=ATOL, rtol=RTOL), np.abs(got - want).max()
     dims = c["after_topology"]
     nsp = len(c["after_net_atomicnumbers"])
     p0, _ = initial_params(dims, nsp, int(c["seed"]))
@@ -90,4 +90,4 @@ def test_oracle_sd_step_from_scratch(name):
         new[sp] = p0[sp] + (step if m <= maxdisp else (maxdisp / m) * step)
     want = c["after_params"]
     err = np.abs(new - want).max()
-    assert err <= ATOL + RTOL * max(1.0, np.abs(want).max()), err
+    assert np.allclose(new, want, atol=ATOL, rtol=RTOL), err
