@@ -1507,7 +1507,9 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
     ctx->grad_cap = np;
   }
   {
-    const long long need = std::max<long long>((long long)b->ntiles, 1) * m.shape.nparams;
+    // per-tile partials, then the per-chunk sums of every species
+    const long long need = std::max<long long>((long long)b->ntiles, 1) * m.shape.nparams +
+                           (long long)grad_chunks(b->ntiles) * m.shape.nparams * m.nspecies;
     if (need > b->gpart_len) {
       dfree(b->gpart);
       b->gpart = nullptr;

@@ -345,6 +345,7 @@ hipError_t launch_sd_step(double* params, const double* grad, long long n, doubl
                           double max_disp, double tol, double* scratch, int nspecies,
                           hipStream_t s);
 int mlp_tile_atoms(const NetShape& sh, int fp64);
+int grad_chunks(int ntiles);  // chunks of the weight-gradient reduction
 
 // launchers (fnet_mlp_mfma.hip): FP32 element networks on MFMA 16x16x4 f32
 int mlp_mfma_waves(const NetShape& sh);  // waves per block, 0 if the shape is not supported

@@ -4,6 +4,8 @@
 // nOut reverse sweeps give the same Jacobian rows), or back through the weights for the
 // training gradient (bprop, network.F90:248-296, with the sysTrain normalisation of
 // bpnn.F90:692-702).  Per-structure energies and weight gradients are reduced in FP64.
+#include <algorithm>
+
 #include "fnet_internal.hpp"
 
 namespace fnet {
@@ -273,17 +275,32 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
   }
 }
 
-// grad[sp][k] = sum over the tiles of species sp, in tile order, of the tiles' partials.
-__global__ void k_grad_reduce(const double* __restrict__ gpart, const int* __restrict__ tile_sp,
+// grad[sp][k] = sum over the tiles of species sp, in tile order, of the tiles' partials, in two
+// fixed-order levels: chunks of kGradChunk tiles (one thread per (chunk, species, k)), then the
+// chunks in order.
+constexpr int kGradChunk = 64;
+
+__global__ void k_grad_chunks(const double* __restrict__ gpart, const int* __restrict__ tile_sp,
                               int ntiles, long long nparams, int nspecies,
-                              double* __restrict__ grad) {
+                              double* __restrict__ gchunk) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nparams * nspecies) return;
+  const int c = blockIdx.y;
   const int sp = (int)(q / nparams);
   const long long k = q - (long long)sp * nparams;
+  const int t0 = c * kGradChunk, t1 = min(ntiles, t0 + kGradChunk);
   double v = 0.0;
-  for (int t = 0; t < ntiles; ++t)
+  for (int t = t0; t < t1; ++t)
     if (tile_sp[t] == sp) v += gpart[(long long)t * nparams + k];
+  gchunk[(long long)c * nparams * nspecies + q] = v;
+}
+
+__global__ void k_grad_reduce(const double* __restrict__ gchunk, int nchunks, long long n,
+                              double* __restrict__ grad) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double v = 0.0;
+  for (int c = 0; c < nchunks; ++c) v += gchunk[(long long)c * n + q];
   grad[q] = v;
 }
 
@@ -525,6 +542,8 @@ MlpArgs make_args(fnet_batch* b, const DevModel& m, int first_tile) {
 
 }  // namespace
 
+int grad_chunks(int ntiles) { return std::max(1, (ntiles + kGradChunk - 1) / kGradChunk); }
+
 int mlp_tile_atoms(const NetShape& sh, int fp64) {
   int TA = 64;
   while (TA > 1 && (fp64 ? host_lds<double>(sh, TA) : host_lds<float>(sh, TA)) > 64 * 1024) TA >>= 1;
@@ -605,8 +624,15 @@ hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* g
       k_mlp_grad<float><<<b->ntiles, 256, host_lds<float>(m.shape, a.TA), s>>>(a);
     }
   }
-  k_grad_reduce<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(b->gpart, b->tile_sp, b->ntiles,
-                                                             m.shape.nparams, m.nspecies, grad);
+  const int nchunks = grad_chunks(b->ntiles);
+  double* gchunk = b->gpart + (long long)std::max(b->ntiles, 1) * m.shape.nparams;
+  if (b->ntiles > 0) {
+    const dim3 grid((unsigned)((np + 255) / 256), (unsigned)nchunks);
+    k_grad_chunks<<<grid, 256, 0, s>>>(b->gpart, b->tile_sp, b->ntiles, m.shape.nparams,
+                                       m.nspecies, gchunk);
+  }
+  k_grad_reduce<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(gchunk, b->ntiles > 0 ? nchunks : 0,
+                                                             np, grad);
   return hipGetLastError();
 }
 
