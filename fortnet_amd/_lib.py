@@ -30,6 +30,7 @@ EXPORTS = [
     "fnet_stage_times", "fnet_batch_reload", "fnet_query", "fnet_set_loss",
     "fnet_set_regularization", "fnet_set_graph", "fnet_batch_sync", "fnet_step_time",
     "fnet_batch_step_info", "fnet_optimizer_init", "fnet_optimizer_step", "fnet_device_memory",
+    "fnet_forces_noalloc",
 ]
 
 
@@ -102,6 +103,7 @@ def load():
         "fnet_optimizer_init": (i32, [vp, i32, dbl, dbl, dbl, dbl, i32, i32]),
         "fnet_optimizer_step": (i32, [vp, i32, vp, ctypes.POINTER(i32)]),
         "fnet_device_memory": (i32, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "fnet_forces_noalloc": (i32, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

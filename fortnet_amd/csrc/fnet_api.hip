@@ -397,11 +397,6 @@ int do_reduce(fnet_batch* b) {
   return FNET_OK;
 }
 
-// Entries of the current list layout: padded rows or CSR.
-long long layout_entries(const fnet_batch* b) {
-  return b->nl_padded ? (long long)b->N * b->nb_cap : b->npairs;
-}
-
 // Per-entry adjoint gradients and mirror entries for the current list layout.
 int ensure_force_buffers(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
@@ -749,6 +744,8 @@ int fnet_ctx_create(int device, fnet_ctx** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FNET_ERR_HIP;
   if (device < 0 || device >= ndev) return FNET_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return FNET_ERR_HIP;
+  // the device's copy of the no-allocation scratch: its address is taken here, once per device
+  if (scratch_init(device) != hipSuccess) return FNET_ERR_HIP;
   fnet_ctx* ctx = new fnet_ctx();
   ctx->device = device;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -23,7 +23,18 @@
 // other.  Such an entry has no mirror: the gather puts it on an orphan list and its -g is added to
 // its neighbour's force afterwards (k_force_orphans, the only atomic update left, for pairs within
 // an ulp of the cutoff).
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "../../include/fnet_hip.h"
 #include "fnet_internal.hpp"
+
+// Scratch of fnet_forces_noalloc: per-entry gradient vectors, mirror entries, the orphan list and
+// its counter of one force assembly, for callers that cannot allocate.  Each GPU holds its own
+// copy until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (scratch_ptr), never through the array's name.
+static __device__ __align__(256) unsigned char scratch[fnet::kNoallocScratchBytes];
 
 namespace fnet {
 namespace {
@@ -200,6 +211,30 @@ hipError_t launch_force_gather(fnet_batch* b, int nout, long long a0, long long 
   return hipGetLastError();
 }
 
+// Per-device address of `scratch`, resolved once per device with that device current.
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 hipError_t launch_force_orphans(fnet_batch* b, int nout, hipStream_t s) {
   const int* nj = b->nl_padded ? b->pnj : b->nj;
   k_force_orphans<<<1, 256, 0, s>>>(b->d_norph, b->orph, nj, b->gvec, nout, (double*)b->F);
@@ -207,3 +242,92 @@ hipError_t launch_force_orphans(fnet_batch* b, int nout, hipStream_t s) {
 }
 
 }  // namespace fnet
+
+using namespace fnet;
+
+namespace {
+
+// The batch's force-assembly buffers pointed at the static scratch for the duration of one
+// enqueue (kernel arguments are captured when launched), restored on every exit.
+struct ScratchSwap {
+  fnet_batch* b;
+  double* gvec;
+  int *mir, *orph, *norph;
+  ScratchSwap(fnet_batch* bb, double* g, int* m, int* o, int* n)
+      : b(bb), gvec(bb->gvec), mir(bb->mir), orph(bb->orph), norph(bb->d_norph) {
+    b->gvec = g;
+    b->mir = m;
+    b->orph = o;
+    b->d_norph = n;
+  }
+  ~ScratchSwap() {
+    b->gvec = gvec;
+    b->mir = mir;
+    b->orph = orph;
+    b->d_norph = norph;
+  }
+};
+
+long long align256(long long x) { return (x + 255) & ~255LL; }
+
+int noalloc_fail(fnet_ctx* ctx, int code, const char* msg) {
+  ctx->err = msg;
+  return code;
+}
+
+}  // namespace
+
+extern "C" int fnet_forces_noalloc(fnet_ctx* ctx, fnet_batch* b) {
+  if (!ctx) return FNET_ERR_ARG;
+  if (!b || b->ctx != ctx) return noalloc_fail(ctx, FNET_ERR_ARG, "batch of another context");
+  const DevModel& m = ctx->model;
+  if (!m.valid) return noalloc_fail(ctx, FNET_ERR_STATE, "no model uploaded");
+  if (!b->have_F || !b->have_nl || b->result_precision != b->precision)
+    return noalloc_fail(ctx, FNET_ERR_STATE,
+                        "fnet_forces_noalloc needs a batch evaluated with forces in its current "
+                        "mode (fnet_run or fnet_forces first)");
+  if (hipSetDevice(ctx->device) != hipSuccess)
+    return noalloc_fail(ctx, FNET_ERR_HIP, "hipSetDevice failed");
+  unsigned char* base = (unsigned char*)scratch_ptr(ctx->device);
+  if (!base) return noalloc_fail(ctx, FNET_ERR_STATE, "device scratch not set up");
+  const long long entries = std::max<long long>(layout_entries(b), 1);
+  const long long gbytes = align256(entries * m.nout * 3 * (long long)sizeof(double));
+  const long long ibytes = align256(entries * (long long)sizeof(int));
+  if (gbytes + 2 * ibytes + 256 > kNoallocScratchBytes)
+    return noalloc_fail(ctx, FNET_ERR_CAPACITY,
+                        "neighbour list too large for the 32 MiB scratch of "
+                        "fnet_forces_noalloc (need (24 n_out + 8) bytes per list entry)");
+  double* gvec = (double*)base;
+  int* mir = (int*)(base + gbytes);
+  int* orph = (int*)(base + gbytes + ibytes);
+  int* norph = (int*)(base + gbytes + 2 * ibytes);
+  const int fp64 = b->precision == FNET_FP64;
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipSuccess;
+  {
+    ScratchSwap sw(b, gvec, mir, orph, norph);
+    if (b->launch == FNET_PER_STRUCT) {
+      for (int st = 0; st < b->S && e == hipSuccess; ++st)
+        e = launch_mirror(b, b->h_off[st], b->h_off[st + 1], st == 0 ? entries : 0, s);
+    } else {
+      e = launch_mirror(b, 0, b->N, entries, s);
+    }
+    if (e == hipSuccess) e = launch_zero_int(norph, 1, s);
+    if (b->launch == FNET_PER_STRUCT) {
+      for (int st = 0; st < b->S && e == hipSuccess; ++st) {
+        e = launch_acsf_adj(b, m, b->h_off[st], b->h_off[st + 1], fp64, s);
+        if (e == hipSuccess) e = launch_force_gather(b, m.nout, b->h_off[st], b->h_off[st + 1], s);
+      }
+    } else {
+      if (e == hipSuccess) e = launch_acsf_adj(b, m, 0, b->N, fp64, s);
+      if (e == hipSuccess) e = launch_force_gather(b, m.nout, 0, b->N, s);
+    }
+    if (e == hipSuccess) e = launch_force_orphans(b, m.nout, s);
+  }
+  if (e != hipSuccess) {
+    ctx->err = std::string("fnet_forces_noalloc: ") + hipGetErrorString(e);
+    return FNET_ERR_HIP;
+  }
+  b->have_F = true;
+  return FNET_OK;
+}

@@ -311,6 +311,15 @@ hipError_t launch_opt_step(OptState* states, double* params, const double* grad,
                            long long n, long long vstride, double fx, int* conv, int nspecies,
                            hipStream_t s);
 
+// entries of a batch's current neighbour-list layout: padded rows or CSR
+inline long long layout_entries(const fnet_batch* b);
+
+// no-allocation force assembly (fnet_gather.hip): static device scratch of kNoallocScratchBytes
+// per GPU, its address resolved per device by scratch_init (called by fnet_ctx_create)
+constexpr long long kNoallocScratchBytes = 32LL << 20;
+hipError_t scratch_init(int device);
+void* scratch_ptr(int device);
+
 // launchers (fnet_gather.hip)
 hipError_t launch_zero_int(int* p, int n, hipStream_t s);
 hipError_t launch_mirror(fnet_batch* b, long long a0, long long a1, long long fill_entries,
@@ -369,3 +378,9 @@ int acsf_slice_entries(int max_nb);  // entries of one wave's LDS slice for rows
     hipError_t _e = (expr);                                  \
     if (_e != hipSuccess) return _e;                         \
   } while (0)
+
+namespace fnet {
+inline long long layout_entries(const fnet_batch* b) {
+  return b->nl_padded ? (long long)b->N * b->nb_cap : b->npairs;
+}
+}  // namespace fnet

@@ -216,6 +216,18 @@ int fnet_optimizer_init(fnet_ctx* ctx, int kind, double tol, double min_disp, do
  * converged receives the flag of the last network, as the reference's training loop reads it. */
 int fnet_optimizer_step(fnet_ctx* ctx, int kind, double* params_out, int* converged);
 
+/* The force assembly of fnet_forces (adjoint pass, mirror entries of the neighbour list and the
+ * per-atom gather) for the batch's resident neighbour list and network input gradients (the
+ * batch must have been evaluated with forces in its current mode: fnet_run or fnet_forces),
+ * writing the batch's force buffer (download with fnet_forces).  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  It allocates nothing, does not synchronise
+ * and does not copy: its per-entry scratch lives in a static device array of 32 MiB per GPU
+ * ((24 n_out + 8) bytes per list entry, larger lists are refused with FNET_ERR_CAPACITY).  All
+ * calls on one GPU share that array: calls that overlap on one GPU (two streams, two contexts)
+ * race. */
+int fnet_forces_noalloc(fnet_ctx* ctx, fnet_batch* batch);
+
 /* Free and total device memory in bytes (hipMemGetInfo): the serving loop admits requests by it.
  * (No reference counterpart.) */
 int fnet_device_memory(fnet_ctx* ctx, long long* free_bytes, long long* total_bytes);
