@@ -1042,10 +1042,48 @@ __device__ __forceinline__ void lad_one(CGroup& grp, CGroup* pg,
   group_lad<T, ADJ, MX, MASK>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, G, nG);
 }
 
+// Forward value of one radial function (G1/G2/G3, acsf.F90:1287-1373) summed over the staged
+// entries e0, e0 + step, ... below e1, with the terms evaluated in U and summed in FP64.  U is the
+// kernel's T, or FP64 in an FP32 kernel whose model standardises some radial feature by a sigma
+// far below its mean (R64, DevModel::radial64): the z-score divides G - mean by that sigma, and the FP32
+// sums' relative error (~4e-7 over a few dozen neighbours) would reach the network input, and
+// through it the forces, amplified by |mean| / sigma.  The FP32 distance is then widened exactly,
+// and the cutoff value is recomputed in FP64 instead of read from the FP32 slice.
+template <typename U, typename T>
+__device__ __forceinline__ double radial_sum(const AcsfArgs& a, const Slice<T>& sl,
+                                             const RadialFn& fn, long long i, int e0, int e1,
+                                             int step) {
+  const bool shared = std::is_same<U, T>::value && fn.rc == a.rclist && fn.atomid == 0;
+  const U pi = (U)kPi;
+  const U rc = (U)fn.rc, eta = (U)fn.eta, rs = (U)fn.rs, kap = (U)fn.kappa;
+  const U pirc = pi / rc;
+  const U qi = (U)ext_q(a.ext, a.next, fn.atomid, i);
+  double acc = 0.0;
+  for (int e = e0; e < e1; e += step) {
+    const U r = (U)sl.d[e].w;
+    if (r <= rc) {
+      U fcv;
+      if (shared) {
+        fcv = (U)sl.fc[e];
+      } else {
+        const U qj = (U)ext_q(a.ext, a.next, fn.atomid, sl.j[e]);
+        fcv = (U)0.5 * qi * qj * (fcos(cutarg(r, pi, rc, pirc)) + (U)1);
+      }
+      U term;
+      if (fn.type == 1) term = fcv;
+      else if (fn.type == 2) term = fexp(-eta * ((r - rs) * (r - rs))) * fcv;
+      else term = fcos(kap * r) * fcv;
+      acc += (double)term;
+    }
+  }
+  return acc;
+}
+
 // LAD: every angular group has the exponent-ladder form (AngGroup::lad > 0); MQ is then unused.
 // PAIRS (adjoint with ladder groups only): paired groups are evaluated together (own instance, so
-// that the kernels of models without pairs keep their register budget).
-template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS>
+// that the kernels of models without pairs keep their register budget).  R64 (FP32 forward
+// only): radial terms in FP64 (radial_sum), also an instance of its own.
+template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS, bool R64 = false>
 __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
@@ -1178,32 +1216,13 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
           act = lane < nf * u;
         }
         const RadialFn& fn = a.radial[f0 + fl];
-        const bool shared = (fn.rc == a.rclist) && fn.atomid == 0;
-        const T rc = (T)fn.rc, eta = (T)fn.eta, rs = (T)fn.rs, kap = (T)fn.kappa;
-        const T pirc = pi / rc;
-        const T qi = (T)ext_q(a.ext, a.next, fn.atomid, i);
         double acc = 0.0;
         if (act) {
           // a species-resolved function sums over its element's range of the staged entries
           const int e0 = fn.zc >= 0 ? sl.st[fn.zc] : 0;
           const int e1 = fn.zc >= 0 ? e0 + sl.nz[fn.zc] : n;
-          for (int e = e0 + gl; e < e1; e += ng) {
-            const T r = sl.d[e].w;
-            if (r <= rc) {
-              T fcv;
-              if (shared) {
-                fcv = sl.fc[e];
-              } else {
-                const T qj = (T)ext_q(a.ext, a.next, fn.atomid, nbr(e));
-                fcv = (T)0.5 * qi * qj * (fcos(cutarg(r, pi, rc, pirc)) + (T)1);
-              }
-              T term;
-              if (fn.type == 1) term = fcv;
-              else if (fn.type == 2) term = fexp(-eta * ((r - rs) * (r - rs))) * fcv;
-              else term = fcos(kap * r) * fcv;
-              acc += (double)term;
-            }
-          }
+          if constexpr (R64) acc = radial_sum<double>(a, sl, fn, i, e0 + gl, e1, ng);
+          else acc = radial_sum<T>(a, sl, fn, i, e0 + gl, e1, ng);
         }
         sl.red[lane] = acc;
         wave_sync();
@@ -1506,6 +1525,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
   acsf_wave<float, true, 1, true, true>(a);
 }
 
+// FP32 forward with the radial terms in FP64 (models with an ill-conditioned radial z-score)
+template <int MQ, bool LAD>
+__global__ void __launch_bounds__(512) k_acsf_fwd32_r64(AcsfArgs a) {
+  acsf_wave<float, false, MQ, LAD, false, true>(a);
+}
+
 template <typename T, bool ADJ, int MQ, bool LAD>
 hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a1,
                      hipStream_t s) {
@@ -1552,6 +1577,12 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
         k_acsf_lad32_pairs<<<grid, 64 * wpb, sb * wpb, s>>>(a);
       else
         k_acsf<T, true, 1, true, true><<<grid, 64 * wpb, sb * wpb, s>>>(a);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (!ADJ && std::is_same<T, float>::value) {
+    if (m.radial64) {
+      k_acsf_fwd32_r64<MQ, LAD><<<grid, 64 * wpb, sb * wpb, s>>>(a);
       return hipGetLastError();
     }
   }

@@ -887,6 +887,10 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
     }
   }
   m.nR = (int)rad.size();
+  m.radial64 = 0;
+  for (const RadialFn& f : rad)
+    if (std::fabs(zmean[f.col]) * zinv[f.col] > kRadial64Kappa)
+      m.radial64 = 1;
   m.radial_resolved = 0;
   for (const RadialFn& f : rad)
     if (f.zc >= 0) m.radial_resolved = 1;

@@ -19,6 +19,13 @@ constexpr int kMaxOut = 8;      // network outputs (targets)
 constexpr double kPi = 3.14159265358979323846; // lib_dftbp/constants.F90:15
 
 // One radial function (G1/G2/G3), acsf.F90:1287-1373.
+// A model whose z-score divides some radial feature by a sigma below |mean| / kRadial64Kappa has
+// its radial functions summed in FP64 in the FP32 modes too.  Summed in FP32 over a few dozen
+// neighbours they carry a relative error of up to ~4e-7, which the z-score turns into ~1e-5 of a
+// network input at this ratio; on a perturbed diamond lattice (|mean| / sigma up to 124) that
+// error alone moved the forces by 1.6e-4 eV/A RMS.
+constexpr double kRadial64Kappa = 25.0;
+
 struct RadialFn {
   int col;     // output column in G
   int type;    // 1, 2, 3
@@ -97,6 +104,7 @@ struct DevModel {
   int all_lad = 0;              // every angular group has the exponent-ladder form
   int has_pairs = 0;            // some angular groups are evaluated in pairs by the adjoint
   int radial_resolved = 0;      // some radial function is restricted to one neighbour element
+  int radial64 = 0;             // FP32 modes sum the radial functions in FP64 (kRadial64Kappa)
   int need_ext = 0;             // external feature columns the model reads
   int zlist[kMaxZ];
   int nspecies = 0, nout = 0, nin = 0, next_in = 0;

@@ -101,14 +101,13 @@ def test_fp64_matches_oracle(ctx, name):
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("precision", [fa.FNET_FP32, fa.FNET_FP32_NET64], ids=["fp32", "fp32net64"])
 def test_fp32_meets_accuracy_gates(ctx, name, precision):
-    """Absolute gates on every workload but the perturbed diamond lattice (si_bulk), which is
-    gated relative to its force scale.  Its features vary by 1-2 % between atoms, so the z-score
-    multiplies dE/dG by 1/sigma ~ 60-120 /|G|: the network input gradient summed over one atom's
-    features reaches |dE/dG|/sigma ~ 5e3, and each FP32 ACSF-derivative term of the adjoint
-    carries that weight.  Measured with the FP64 oracle, rounding G alone to FP32 moves the forces
-    by 1.3e-5 eV/A RMS and rounding the coordinates by 8.9e-6; the FP32 adjoint terms bring the
-    total to ~1.9e-4 eV/A, 3e-5 of the force RMS of 6.7 eV/A.  That is not an error of the
-    networks: FP64 networks (FNET_FP32_NET64) give the same figure."""
+    """Absolute gates on every workload.  The perturbed lattices (si64, si_bulk) standardise
+    radial features whose spread between atoms is 0.8-2 % of their mean: the z-score divides
+    G - mean by that sigma, so the FP32 radial sums' own error (~4e-7 relative) reached the network
+    input 50-120 times larger and moved the si_bulk forces by 1.6e-4 eV/A RMS (measured with a
+    per-term FP32 restatement on the CPU; the device gave 1.9e-4).  Such models (|mean| / sigma
+    above 25 for some radial feature) have their radial sums evaluated in FP64; the FP32 angular
+    sums, the FP32 adjoint and the FP32 pair vectors then leave ~6e-5 eV/A."""
     print("workload", name, "precision", precision, flush=True)
     geos, ns = _setup(ctx, name)
     b = ctx.batch(geos, fa.FNET_FP64)
@@ -124,11 +123,7 @@ def test_fp32_meets_accuracy_gates(ctx, name, precision):
           % (name, precision, e_err.max(), f_rmse))
     b.close()
     assert e_err.max() <= E_TOL
-    if name == "si_bulk":
-        f_rms = np.sqrt(np.mean(F64 ** 2)) * EV_BOHR_TO_EV_AA
-        assert f_rmse <= 1e-4 * f_rms
-    else:
-        assert f_rmse <= F_RMSE_TOL
+    assert f_rmse <= F_RMSE_TOL
 
 
 @pytest.mark.parametrize("name", ["si64", "water"])
