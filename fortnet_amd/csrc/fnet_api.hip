@@ -1468,9 +1468,10 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   rc = do_reduce(b);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  // stage inputs: gt [S*ng], at [N*na], aw [N], dpw [S]
+  // stage inputs: gt [S*ng], at [N*na], aw [N], dpw [S]; then the loss, a pad and the
+  // per-structure loss terms [2 S]
   const long long na = m.nout - n_global;
-  const long long need = (long long)b->S * n_global + b->N * na + b->N + b->S + 4;
+  const long long need = (long long)b->S * n_global + b->N * na + b->N + 3LL * b->S + 4;
   if (need > b->targets_cap) {
     dfree(b->d_targets);
     b->d_targets = nullptr;
@@ -1509,8 +1510,9 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   }
   {
     // per-tile partials, then the per-chunk sums of every species
-    const long long need = std::max<long long>((long long)b->ntiles, 1) * m.shape.nparams +
-                           (long long)grad_chunks(b->ntiles) * m.shape.nparams * m.nspecies;
+    const int parts = grad_parts(b, m, net64(b->precision));
+    const long long need = std::max<long long>((long long)parts, 1) * m.shape.nparams +
+                           (long long)grad_chunks(parts) * m.shape.nparams * m.nspecies;
     if (need > b->gpart_len) {
       dfree(b->gpart);
       b->gpart = nullptr;
@@ -1523,7 +1525,7 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
     HIPCHECK(ctx, launch_regularize(m, ctx->d_grad, ctx->reg_strength / wsum, ctx->reg_alpha, s));
   double* dloss = dpw + b->S;
   HIPCHECK(ctx, launch_loss(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
-                            dp_weights ? dpw : nullptr, ctx->loss, dloss, s));
+                            dp_weights ? dpw : nullptr, ctx->loss, dloss, dloss + 2, s));
   if (grad)
     HIPCHECK(ctx, hipMemcpyAsync(grad, ctx->d_grad, sizeof(double) * np, hipMemcpyDeviceToHost, s));
   if (pred)

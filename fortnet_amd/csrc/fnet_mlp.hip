@@ -280,18 +280,21 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpArgs a) {
 // chunks in order.
 constexpr int kGradChunk = 64;
 
-__global__ void k_grad_chunks(const double* __restrict__ gpart, const int* __restrict__ tile_sp,
-                              int ntiles, long long nparams, int nspecies,
+// Partial t belongs to tile t / nsub (the MFMA path writes nsub FP32 partials per tile; their
+// values are FP32 sums, so FP32 storage loses nothing, and they are added in FP64 from here on).
+template <typename P>
+__global__ void k_grad_chunks(const P* __restrict__ gpart, const int* __restrict__ tile_sp,
+                              int nparts, int nsub, long long nparams, int nspecies,
                               double* __restrict__ gchunk) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nparams * nspecies) return;
   const int c = blockIdx.y;
   const int sp = (int)(q / nparams);
   const long long k = q - (long long)sp * nparams;
-  const int t0 = c * kGradChunk, t1 = min(ntiles, t0 + kGradChunk);
+  const int t0 = c * kGradChunk, t1 = min(nparts, t0 + kGradChunk);
   double v = 0.0;
   for (int t = t0; t < t1; ++t)
-    if (tile_sp[t] == sp) v += gpart[(long long)t * nparams + k];
+    if (tile_sp[t / nsub] == sp) v += (double)gpart[(long long)t * nparams + k];
   gchunk[(long long)c * nparams * nspecies + q] = v;
 }
 
@@ -394,51 +397,61 @@ __device__ __forceinline__ double item_loss(int loss, double sum_sq, double sum_
   }
 }
 
-// Loss of the batch (maLoss / mapLoss / msLoss / rmsLoss, loss.F90:370-700) in FP64, one block:
-// global targets weighted by w_s * sum(atomic weights of s), atomic targets by w_s * atomic
-// weight; the result is divided by the summed weights.
-__global__ void __launch_bounds__(256) k_loss(const double* __restrict__ Eatom,
-                                              const double* __restrict__ Estruct,
-                                              const long long* __restrict__ off, int S, int nout,
-                                              int nglob, const double* __restrict__ gt,
-                                              const double* __restrict__ at,
-                                              const double* __restrict__ aw,
-                                              const double* __restrict__ dpw, int loss,
-                                              double* __restrict__ out) {
-  __shared__ double rl[256], rn[256];
+// Loss of the batch (maLoss / mapLoss / msLoss / rmsLoss, loss.F90:370-700) in FP64: global
+// targets weighted by w_s * sum(atomic weights of s), atomic targets by w_s * atomic weight; the
+// result is divided by the summed weights.  k_loss_terms: one thread per structure writes its
+// weighted loss and weight; k_loss_sum adds them in a fixed order (thread t takes structures
+// t, t + 256, ... in order, then a fixed tree).
+__global__ void k_loss_terms(const double* __restrict__ Eatom, const double* __restrict__ Estruct,
+                             const long long* __restrict__ off, int S, int nout, int nglob,
+                             const double* __restrict__ gt, const double* __restrict__ at,
+                             const double* __restrict__ aw, const double* __restrict__ dpw,
+                             int loss, double* __restrict__ terms) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
   double l = 0.0, nv = 0.0;
   const int natomic = nout - nglob;
-  for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    const double w = dpw ? dpw[s] : 1.0;
-    double saw = 0.0;
-    for (long long i = off[s]; i < off[s + 1]; ++i) {
-      const double awi = aw ? aw[i] : 1.0;
-      saw += awi;
-      if (natomic > 0) {
-        double q = 0.0, a = 0.0, r = 0.0;
-        for (int o = 0; o < natomic; ++o) {
-          const double tv = at[i * natomic + o];
-          const double d = tv - Eatom[i * nout + nglob + o];
-          q += d * d;
-          a += fabs(d);
-          r += fabs(d / tv);
-        }
-        l += w * awi * item_loss(loss, q, a, r, natomic);
-        nv += w * awi;
-      }
-    }
-    if (nglob > 0) {
+  const double w = dpw ? dpw[s] : 1.0;
+  double saw = 0.0;
+  for (long long i = off[s]; i < off[s + 1]; ++i) {
+    const double awi = aw ? aw[i] : 1.0;
+    saw += awi;
+    if (natomic > 0) {
       double q = 0.0, a = 0.0, r = 0.0;
-      for (int o = 0; o < nglob; ++o) {
-        const double tv = gt[(long long)s * nglob + o];
-        const double d = tv - Estruct[(long long)s * nout + o];
+      for (int o = 0; o < natomic; ++o) {
+        const double tv = at[i * natomic + o];
+        const double d = tv - Eatom[i * nout + nglob + o];
         q += d * d;
         a += fabs(d);
         r += fabs(d / tv);
       }
-      l += w * saw * item_loss(loss, q, a, r, nglob);
-      nv += w * saw;
+      l += w * awi * item_loss(loss, q, a, r, natomic);
+      nv += w * awi;
     }
+  }
+  if (nglob > 0) {
+    double q = 0.0, a = 0.0, r = 0.0;
+    for (int o = 0; o < nglob; ++o) {
+      const double tv = gt[(long long)s * nglob + o];
+      const double d = tv - Estruct[(long long)s * nout + o];
+      q += d * d;
+      a += fabs(d);
+      r += fabs(d / tv);
+    }
+    l += w * saw * item_loss(loss, q, a, r, nglob);
+    nv += w * saw;
+  }
+  terms[2 * (long long)s] = l;
+  terms[2 * (long long)s + 1] = nv;
+}
+
+__global__ void __launch_bounds__(256) k_loss_sum(const double* __restrict__ terms, int S,
+                                                  double* __restrict__ out) {
+  __shared__ double rl[256], rn[256];
+  double l = 0.0, nv = 0.0;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    l += terms[2 * (long long)s];
+    nv += terms[2 * (long long)s + 1];
   }
   rl[threadIdx.x] = l;
   rn[threadIdx.x] = nv;
@@ -592,9 +605,11 @@ hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const
 
 hipError_t launch_loss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                        const double* at, const double* aw, const double* dpw, int loss,
-                       double* out, hipStream_t s) {
-  k_loss<<<1, 256, 0, s>>>(b->Eatom, b->Estruct, b->off, b->S, m.nout, n_global, gt, at, aw, dpw,
-                           loss, out);
+                       double* out, double* terms, hipStream_t s) {
+  if (b->S > 0)
+    k_loss_terms<<<(b->S + 255) / 256, 256, 0, s>>>(b->Eatom, b->Estruct, b->off, b->S, m.nout,
+                                                    n_global, gt, at, aw, dpw, loss, terms);
+  k_loss_sum<<<1, 256, 0, s>>>(terms, b->S, out);
   return hipGetLastError();
 }
 
@@ -610,10 +625,20 @@ hipError_t launch_regularize(const DevModel& m, double* grad, double scale, doub
   return hipGetLastError();
 }
 
+int grad_parts(const fnet_batch* b, const DevModel& m, int fp64) {
+  const int nsub = fp64 || !m.wpack ? 0 : mlp_grad_subtiles(m.shape, b->mtile_waves);
+  return nsub ? b->nmtiles * nsub : b->ntiles;
+}
+
 hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* grad,
                            hipStream_t s) {
   const long long np = m.shape.nparams * m.nspecies;
-  if (b->ntiles > 0) {
+  const int nsub = fp64 || !m.wpack ? 0 : mlp_grad_subtiles(m.shape, b->mtile_waves);
+  const int nparts = nsub ? b->nmtiles * nsub : b->ntiles;
+  if (nsub) {
+    const hipError_t e = launch_mlp_grad_mfma(b, m, s);
+    if (e != hipSuccess) return e;
+  } else if (b->ntiles > 0) {
     if (fp64) {
       MlpArgs a = make_args<double>(b, m, 0);
       a.grad = b->gpart;
@@ -624,15 +649,19 @@ hipError_t launch_mlp_grad(fnet_batch* b, const DevModel& m, int fp64, double* g
       k_mlp_grad<float><<<b->ntiles, 256, host_lds<float>(m.shape, a.TA), s>>>(a);
     }
   }
-  const int nchunks = grad_chunks(b->ntiles);
-  double* gchunk = b->gpart + (long long)std::max(b->ntiles, 1) * m.shape.nparams;
-  if (b->ntiles > 0) {
+  const int nchunks = grad_chunks(nparts);
+  double* gchunk = b->gpart + (long long)std::max(nparts, 1) * m.shape.nparams;
+  if (nparts > 0) {
     const dim3 grid((unsigned)((np + 255) / 256), (unsigned)nchunks);
-    k_grad_chunks<<<grid, 256, 0, s>>>(b->gpart, b->tile_sp, b->ntiles, m.shape.nparams,
-                                       m.nspecies, gchunk);
+    if (nsub)
+      k_grad_chunks<float><<<grid, 256, 0, s>>>((const float*)b->gpart, b->mtile_sp, nparts, nsub,
+                                                m.shape.nparams, m.nspecies, gchunk);
+    else
+      k_grad_chunks<double><<<grid, 256, 0, s>>>(b->gpart, b->tile_sp, nparts, 1,
+                                                 m.shape.nparams, m.nspecies, gchunk);
   }
-  k_grad_reduce<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(gchunk, b->ntiles > 0 ? nchunks : 0,
-                                                             np, grad);
+  k_grad_reduce<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(gchunk, nparts > 0 ? nchunks : 0, np,
+                                                             grad);
   return hipGetLastError();
 }
 

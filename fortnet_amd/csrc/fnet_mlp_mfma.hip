@@ -328,6 +328,214 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
   }
 }
 
+// ---- weight gradient (bprop, network.F90:248-296, with sysTrain's scaling, bpnn.F90:692-702) ----
+// One block = nw waves = 16 nw atoms of one element (a tile of the MFMA layout, or a fixed part of
+// one).  Forward: every wave runs its 16 atoms through the hidden layers as above, keeping each
+// layer's pre-activation rows z_l for all atoms of the block in LDS (R[l]; R[0] holds the input
+// rows) and its own current activations in Xw.  Backward, top layer first, with delta_out = the
+// loss gradient of each atom (nout = 1):
+//  * the output layer (rank one) on the vector unit: dW[0][k] = sum_at delta_out a_l[at][k],
+//    delta_l = W[0][k] delta_out f'(z_l);
+//  * every other layer l: dW_l[o][k] = sum_at delta_{l+1}[at][o] a_l[at][k] on the matrix cores
+//    with the block's atoms as the K dimension (A = delta rows, B = a_l = f(z_l) rows; a wave
+//    takes a k-block with up to 8 o-blocks, so each element is activated once per 8 o-blocks), the bias
+//    gradient as a column sum in atom order, then delta_l = (W_l^T delta_{l+1}) f'(z_l) with the
+//    reverse fragments, written over z_l.
+// Products and sums over the block's atoms are FP32 (column sums over the atoms in four fixed
+// quarters, then the quarters in order); the block writes its FP32 partial vector
+// (gpart + block * nparams), which k_grad_chunks / k_grad_reduce add in FP64 in a fixed order.
+struct GradArgs {
+  const double* lossgrad;  // [N][nout]
+  float* gpart;            // [blocks][nparams]
+  int nsub;                // blocks per tile
+  int roff[kMaxLayers];    // LDS offsets (floats) of R[l], l = 0 .. nl - 2
+  int ld[kMaxLayers];      // their row strides
+  int doff;                // delta_out [16 nw]
+  int woff;                // Xw of wave 0 (16 rows of stride ldw per wave); after the forward,
+                           // scratch of the column sums (4 r32(widest layer) floats)
+  int ldw;
+};
+
+// out[k] (k < n) = sum over the block's TA atoms of v(at, k) = wt[at] M[at][k] (ACT: f(M)), in
+// four quarters of the atoms, each in order, then ((q0 + q1) + (q2 + q3)).  Ends synchronised.
+template <bool WT, bool ACT>
+__device__ __forceinline__ void col_sums(const float* M, int ld, int n, int TA, const float* wt,
+                                         int act, float* red, float* out_f) {
+  const int qa = TA >> 2;
+  for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) {
+    const int q = idx / n, k = idx - q * n;
+    float s = 0.0f;
+    for (int at = q * qa; at < (q + 1) * qa; ++at) {
+      float v = M[at * ld + k];
+      if (ACT) v = actf(act, v);
+      s += WT ? wt[at] * v : v;
+    }
+    red[idx] = s;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x)
+    out_f[k] = (red[k] + red[n + k]) + (red[2 * n + k] + red[3 * n + k]);
+  __syncthreads();
+}
+
+template <int BF>
+__global__ void __launch_bounds__(256) k_mlp_grad_mfma(MfmaArgs a, GradArgs ga) {
+  extern __shared__ __align__(16) float smf[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int TA = 16 * nw;
+  const int t = a.first_tile + (int)blockIdx.x / ga.nsub, sub = (int)blockIdx.x % ga.nsub;
+  const int sp = a.tile_sp[t];
+  const int start = a.tile_start[t] + sub * TA;
+  const int cnt = min(TA, a.tile_cnt[t] - sub * TA);  // may be <= 0: the partial is then zero
+  const int nl = a.sh.nl;
+  const int act = a.sh.act[sp];
+  const float* WP = a.wpack + (long long)sp * a.pk_stride;
+  float* gp = ga.gpart + (long long)blockIdx.x * a.sh.nparams;
+  float* dout = smf + ga.doff;
+  float* Xw = smf + ga.woff + w * 16 * ga.ldw;
+  float* red = smf + ga.woff;
+  const int a0 = 16 * w;
+
+  // entries without gradient: the dummy output weights and the input layer's biases
+  for (int k = threadIdx.x; k < a.sh.dims[nl - 1]; k += blockDim.x) gp[a.sh.woff[nl - 1] + k] = 0.0f;
+  for (int k = threadIdx.x; k < a.sh.dims[0]; k += blockDim.x) gp[a.sh.boff[0] + k] = 0.0f;
+
+  // input rows (own atoms) and delta_out
+  {
+    float* X0 = smf + ga.roff[0] + a0 * ga.ld[0];
+    const int d0 = a.sh.dims[0], K0 = r32(d0);
+    for (int idx = lane; idx < 16 * K0; idx += 64) {
+      const int r = idx / K0, k = idx - r * K0;
+      float v = 0.0f;
+      if (a0 + r < cnt && k < d0) {
+        const long long at = a.perm[start + a0 + r];
+        v = k < a.nG ? (float)((a.G[at * a.nG + k] - a.zmean[k]) * a.zinv[k])
+                     : (float)a.ext[at * a.next + a.ext_in[k - a.nG]];
+      }
+      X0[r * ga.ld[0] + k] = v;
+    }
+    if (lane < 16)
+      dout[a0 + lane] =
+          a0 + lane < cnt ? (float)ga.lossgrad[(long long)a.perm[start + a0 + lane] * a.nout] : 0.0f;
+  }
+  wave_sync();
+
+  // forward through the hidden layers: z_{l+1} rows into R[l+1], f(z_{l+1}) into Xw
+  for (int l = 0; l + 2 < nl; ++l) {
+    const int Kc = r32(a.sh.dims[l]) >> 4, Nb = r32(a.sh.dims[l + 1]) >> 4;
+    const float* Xin = l == 0 ? smf + ga.roff[0] + a0 * ga.ld[0] : Xw;
+    const int ldin = l == 0 ? ga.ld[0] : ga.ldw;
+    const int ldz = ga.ld[l + 1];
+    float* Z = smf + ga.roff[l + 1] + a0 * ldz;
+    const float* Bp = WP + a.pb[l];
+    const bool bf = BF && hidden_hidden(l, nl);
+    for (int ob0 = 0; ob0 < Nb; ob0 += 2) {
+      f4 c0 = *(const f4*)(Bp + 16 * ob0 + 4 * g);
+      f4 c1 = *(const f4*)(Bp + 16 * ob0 + 16 + 4 * g);
+      if (bf)
+        gemm_pair_bf<BF>(a.bpack + (long long)sp * a.bk_stride + a.bfw[l], a.bsz[l], ob0, Kc >> 1,
+                         Xin, ldin, lane, j, g, c0, c1);
+      else
+        gemm_pair(WP + a.fw[l], ob0, Kc, Xin, ldin, lane, j, g, c0, c1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Z[j * ldz + 16 * ob0 + 4 * g + r] = c0[r];
+        Z[j * ldz + 16 * ob0 + 16 + 4 * g + r] = c1[r];
+      }
+    }
+    wave_sync();
+    const int W = 16 * Nb;
+    for (int idx = lane; idx < 16 * W; idx += 64) {
+      const int r = idx / W, k = idx - r * W;
+      Xw[r * ga.ldw + k] = actf(act, Z[r * ldz + k]);
+    }
+    wave_sync();
+  }
+  __syncthreads();  // the backward reads every wave's rows
+
+  // output layer (one output): dW[0][k], the bias, and delta of the last hidden layer
+  {
+    const int l = nl - 2, dl = a.sh.dims[l], ldl = ga.ld[l];
+    float* Z = smf + ga.roff[l];
+    if (l == 0) col_sums<true, false>(Z, ldl, dl, TA, dout, act, red, gp + a.sh.woff[l]);
+    else col_sums<true, true>(Z, ldl, dl, TA, dout, act, red, gp + a.sh.woff[l]);
+    col_sums<false, false>(dout, 1, 1, TA, nullptr, act, red, gp + a.sh.boff[nl - 1]);
+    if (l >= 1) {
+      const float* col = a.params_f + (long long)sp * a.sh.nparams + a.sh.woff[l];
+      for (int idx = threadIdx.x; idx < TA * dl; idx += blockDim.x) {
+        const int at = idx / dl, k = idx - at * dl;
+        float* zp = Z + at * ldl + k;
+        *zp = col[k] * dout[at] * actd(act, *zp);
+      }
+      __syncthreads();
+    }
+  }
+
+  // the other layers, top down: R[l + 1] holds delta_{l+1}, R[l] holds z_l (x for l = 0)
+  for (int l = nl - 3; l >= 0; --l) {
+    const int dl = a.sh.dims[l], dn = a.sh.dims[l + 1];
+    const int Kb = r32(dl) >> 4, Ob = r32(dn) >> 4;
+    const int ldl = ga.ld[l], ldn = ga.ld[l + 1];
+    float* Zl = smf + ga.roff[l];
+    const float* Dn = smf + ga.roff[l + 1];
+    // work items (k-block, group of up to 8 o-blocks); groups narrower than 8 when there are
+    // fewer k-blocks than waves (the input layer), so that every wave has one
+    const int gob = Kb >= nw ? 8 : max(1, min(8, (Ob * Kb) / nw));
+    const int ngr = (Ob + gob - 1) / gob;
+    for (int it = w; it < Kb * ngr; it += nw) {
+      const int kb = it / ngr;
+      {
+        const int ob0 = (it - kb * ngr) * gob, ob1 = min(Ob, ob0 + gob);
+        f4 c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+        for (int s4 = 0; s4 < TA; s4 += 4) {
+          const int at = s4 + g;
+          const float z = Zl[at * ldl + 16 * kb + j];
+          const float bv = l == 0 ? z : actf(act, z);
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (ob0 + q < ob1)
+              c[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Dn[at * ldn + 16 * (ob0 + q) + j], bv,
+                                                          c[q], 0, 0, 0);
+        }
+        const int k = 16 * kb + j;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (ob0 + q < ob1)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int o = 16 * (ob0 + q) + 4 * g + r;
+              if (o < dn && k < dl) gp[a.sh.woff[l] + (long long)o * dl + k] = c[q][r];
+            }
+      }
+    }
+    col_sums<false, false>(Dn, ldn, dn, TA, nullptr, act, red, gp + a.sh.boff[l + 1]);
+    if (l == 0) break;
+    // delta_l = (W_l^T delta_{l+1}) f'(z_l), own atoms, over z_l
+    const int Mb = Kb, Kc = Ob;
+    const bool bf = BF && hidden_hidden(l, nl);
+    float* zr = Zl + (a0 + j) * ldl;
+    for (int ib0 = 0; ib0 < Mb; ib0 += 2) {
+      f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (bf)
+        gemm_pair_bf<BF>(a.bpack + (long long)sp * a.bk_stride + a.brw[l], a.bsz[l], ib0, Kc >> 1,
+                         Dn + a0 * ldn, ldn, lane, j, g, c0, c1);
+      else
+        gemm_pair(WP + a.rw[l], ib0, Kc, Dn + a0 * ldn, ldn, lane, j, g, c0, c1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k0 = 16 * ib0 + 4 * g + r;
+        zr[k0] = c0[r] * actd(act, zr[k0]);
+        zr[k0 + 16] = c1[r] * actd(act, zr[k0 + 16]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 int mlp_mfma_waves(const NetShape& sh) {
@@ -463,6 +671,92 @@ void pack_bf16_weights(const NetShape& sh, int nspecies, const double* params,
             }
     }
   }
+}
+
+namespace {
+// Waves per weight-gradient block (the largest divisor of the tile's waves whose LDS fits 64 KiB),
+// with the LDS layout of k_mlp_grad_mfma; 0 when the network has no MFMA tiles.
+int mlp_grad_waves(const NetShape& sh, int tile_waves, GradArgs* ga, size_t* bytes) {
+  if (tile_waves <= 0) return 0;
+  for (int nw = tile_waves; nw >= 1; nw >>= 1) {
+    if (tile_waves % nw) continue;
+    const int TA = 16 * nw;
+    GradArgs q{};
+    int f = 0;
+    for (int l = 0; l + 1 < sh.nl; ++l) {
+      q.ld[l] = r32(sh.dims[l]) + 4;
+      q.roff[l] = f;
+      f += TA * q.ld[l];
+    }
+    q.doff = f;
+    f += TA;
+    q.ldw = 0;
+    int wmax = 0;
+    for (int l = 0; l < sh.nl; ++l) wmax = std::max(wmax, r32(sh.dims[l]));
+    for (int l = 1; l + 1 < sh.nl; ++l) q.ldw = std::max(q.ldw, r32(sh.dims[l]) + 4);
+    q.woff = f;
+    f += std::max(nw * 16 * q.ldw, 4 * wmax);
+    const size_t b = (size_t)f * sizeof(float);
+    if (b <= 64 * 1024) {
+      q.nsub = tile_waves / nw;
+      if (ga) *ga = q;
+      if (bytes) *bytes = b;
+      return nw;
+    }
+  }
+  return 0;
+}
+}  // namespace
+
+// Weight-gradient blocks per MFMA tile (0: the network has no MFMA path)
+int mlp_grad_subtiles(const NetShape& sh, int tile_waves) {
+  GradArgs ga;
+  return mlp_grad_waves(sh, tile_waves, &ga, nullptr) ? ga.nsub : 0;
+}
+
+hipError_t launch_mlp_grad_mfma(fnet_batch* b, const DevModel& m, hipStream_t s) {
+  GradArgs ga;
+  size_t lds = 0;
+  const int nw = mlp_grad_waves(m.shape, b->mtile_waves, &ga, &lds);
+  if (nw == 0) return hipErrorInvalidConfiguration;
+  if (b->nmtiles <= 0) return hipSuccess;
+  MfmaArgs a{};
+  a.sh = m.shape;
+  a.wpack = m.wpack;
+  a.pk_stride = m.pk_stride;
+  for (int l = 0; l < kMaxLayers; ++l) {
+    a.fw[l] = m.pk_fw[l];
+    a.rw[l] = m.pk_rw[l];
+    a.pb[l] = m.pk_b[l];
+    a.bfw[l] = m.bk_fw[l];
+    a.brw[l] = m.bk_rw[l];
+    a.bsz[l] = m.bk_sz[l];
+  }
+  a.params_f = m.params_f;
+  a.G = (const double*)b->G;
+  a.ext = b->ext;
+  a.next = b->next;
+  a.ext_in = m.ext_in;
+  a.nG = m.nG;
+  a.nout = m.nout;
+  a.zmean = m.zmean;
+  a.zinv = m.zinv;
+  a.perm = b->perm;
+  a.tile_sp = b->mtile_sp;
+  a.tile_start = b->mtile_start;
+  a.tile_cnt = b->mtile_cnt;
+  a.first_tile = 0;
+  a.bpack = (const bf8*)m.bpack;
+  a.bk_stride = m.bk_stride;
+  ga.lossgrad = b->lossgrad;
+  ga.gpart = (float*)b->gpart;
+  const int bf = (b->precision == FNET_BF16 && m.bpack) ? 3
+               : (b->precision == FNET_BF16_FAST && m.bpack) ? 1 : 0;
+  const dim3 grid((unsigned)((long long)b->nmtiles * ga.nsub)), block(64 * nw);
+  if (bf == 3) k_mlp_grad_mfma<3><<<grid, block, lds, s>>>(a, ga);
+  else if (bf == 1) k_mlp_grad_mfma<1><<<grid, block, lds, s>>>(a, ga);
+  else k_mlp_grad_mfma<0><<<grid, block, lds, s>>>(a, ga);
+  return hipGetLastError();
 }
 
 hipError_t launch_mlp_mfma(fnet_batch* b, const DevModel& m, int first_tile, int ntiles,

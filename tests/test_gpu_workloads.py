@@ -154,6 +154,41 @@ def test_train_gradients_match_oracle(ctx, name):
         b.close()
 
 
+@pytest.mark.parametrize("name,scale", [("si_cells", 0.004), ("si_bulk", 0.125),
+                                        ("ternary", 0.03), ("water", 0.004)])
+def test_train_gradients_precisions(ctx, name, scale):
+    """Weight gradients of the FP32 and BF16 modes (matrix-core kernel: 64-atom blocks on the
+    24-64-64-1 and 48-64-64-1 networks, 32-atom halves of a tile on 32-128-128-1) against the
+    FP64 gradients of the same batch: FP32 and split BF16 within 1e-3 of the largest component,
+    single-pass BF16 within 5e-2 (reported, not gated).  Repeated calls give the same bits."""
+    print("workload", name, scale, flush=True)
+    geos, ns, _ = W.config(name, scale)
+    if name == "ternary":
+        geos = [g for g in geos if g.n <= 600][:4]
+    ctx.upload_model(ns)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    ns = W.with_zscore(ns, b.acsf())
+    b.close()
+    ctx.upload_model(ns)
+    rng = np.random.default_rng(1)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    Es, _ = b.energy()
+    targets = Es[:, :1] + rng.normal(size=(len(geos), 1)) * np.array([[g.n for g in geos]]).T * 1e-2
+    g64, _ = b.train_gradients(targets)
+    scale_g = np.abs(g64).max()
+    errs = {}
+    for prec, tol in ((fa.FNET_FP32, 1e-3), (fa.FNET_BF16, 1e-3), (fa.FNET_BF16_FAST, 5e-2)):
+        b.set_mode(prec)
+        g1, _ = b.train_gradients(targets)
+        g2, _ = b.train_gradients(targets)
+        assert np.array_equal(g1, g2)
+        errs[prec] = np.abs(g1 - g64).max() / scale_g
+        assert errs[prec] <= tol, (prec, errs[prec])
+    print("%s gradient max err / max |g|: fp32 %.2e bf16 %.2e bf16_fast %.2e"
+          % (name, errs[fa.FNET_FP32], errs[fa.FNET_BF16], errs[fa.FNET_BF16_FAST]))
+    b.close()
+
+
 @pytest.mark.parametrize("name", ["si64", "water", "ternary"])
 def test_generic_exponent_path(ctx, name):
     """One more angular function with xi = 3 (same type, cutoff, eta, species and atom id as the
