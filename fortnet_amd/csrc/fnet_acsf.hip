@@ -163,7 +163,9 @@ struct AcsfArgs {
   const double* cG; // [N][nout][nG] (adjoint)
   double* F;        // [N][nout][3] (adjoint): the centre's own sum of its entries' gradients
   double* gvec;     // [entries][nout][3] (adjoint): gradient of E with respect to each entry's
-                    // image position, at the entry's index in the list layout
+                    // image position, stored at the index of the entry's mirror (the entry of
+                    // the neighbour's row that points back), or at its own index without one
+  const int* mir;   // [entries] mirror entries (-1: none)
 };
 
 // Bytes of one wave's LDS slice (host and device agree on this layout).  Per entry, in element
@@ -1475,8 +1477,10 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
 
     // ---- output of the adjoint: per-entry gradients and the centre's sum ----
     // The neighbour side (-g of every entry whose neighbour is an atom) is gathered per atom
-    // afterwards through the mirror entries (k_force_gather), in a fixed order and without
-    // atomics; the centre side (+sum of the centre's entries) is written here.
+    // afterwards (k_force_gather), in a fixed order and without atomics: g goes to the slot of
+    // the entry's mirror, in the neighbour's own row, so that the gather reads its row
+    // contiguously instead of one scattered 24-byte record per entry.  The centre side (+sum of
+    // the centre's entries) is written here.
     if (ADJ) {
       double sx = 0, sy = 0, sz = 0;
       for (int e0 = 0; e0 < n; e0 += 64) {
@@ -1484,7 +1488,9 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
         if (e >= n) break;
         const double gx = (double)sl.gs[3 * e], gy = (double)sl.gs[3 * e + 1],
                      gz = (double)sl.gs[3 * e + 2];
-        double* gv = a.gvec + ((b + sl.j[e]) * a.nout + t) * 3;
+        const long long own = b + sl.j[e];
+        const int me = a.mir[own];
+        double* gv = a.gvec + ((me >= 0 ? (long long)me : own) * a.nout + t) * 3;
         gv[0] = gx;
         gv[1] = gy;
         gv[2] = gz;
@@ -1570,6 +1576,7 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
   a.gvec = b->gvec;
+  a.mir = b->mir;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
   if constexpr (ADJ && LAD) {
     if (pairs) {
