@@ -6,13 +6,16 @@
 !>
 !> Case file (list-directed, in this order): precision (0 FP64, 1 FP32, 2 FP32 with FP64
 !> networks); nG; nG lines "type rcut eta rs kappa lambda xi atomid z1 z2"; 1 and the nG means
-!> and nG standard deviations of the z-score, or 0; nSpecies; the atomic number of each network;
-!> the activation of each network; nLayers; the layer sizes; nParams; nParams x nSpecies
-!> parameters, network after network; nStruct; per structure "nAtom periodic", the lattice
-!> vectors (9 values, vector after vector), then "x y z atomicNumber species" per atom (Bohr,
-!> 1-based species); finally 1 followed by the targets (nOut x nStruct), the atomic weights
-!> (nAtom), the datapoint weights (nStruct) and the learning rate, maximal displacement and
-!> tolerance of one steepest-descent step, or 0.
+!> and nG standard deviations of the z-score, or 0; nExtIn and (if nExtIn > 0) the 1-based
+!> indices of the external features the networks read; nSpecies; the atomic number of each
+!> network; the activation of each network; nLayers; the layer sizes; nParams; nParams x
+!> nSpecies parameters, network after network; nStruct; nExt (external features per atom); per
+!> structure "nAtom periodic", the lattice vectors (9 values, vector after vector), then
+!> "x y z atomicNumber species ext(1:nExt)" per atom (Bohr, 1-based species); finally 1
+!> followed by nGlobal, the global targets (nGlobal x nStruct), the atomic targets
+!> ((nOut - nGlobal) x nAtom), the atomic weights (nAtom), the datapoint weights (nStruct) and
+!> the learning rate, maximal displacement and tolerance of one steepest-descent step, or 0.
+!> Empty arrays have no line.
 !>
 !> Result file: blocks "NAME n1 n2" followed by n1 x n2 values, one per line, in Fortran order:
 !> G (nG, nAtom), E (nOut, nStruct), A (nOut, nAtom), F (3 nOut, nAtom), and with a training
@@ -26,8 +29,9 @@ program fnet_fortran_check
   type(TFnetGpu) :: gpu
   character(len=4096) :: caseFile, resultFile, arg
   integer :: fd, fo, precision, ii, iS, iA, nS, nA, nL, nP, per, hasZ, hasTrain, device, nOut
-  real(dp), allocatable :: vals(:,:), atomic(:,:), sums(:,:), forces(:,:), targets(:,:), &
-      & grad(:,:), params(:,:), atomicWeights(:), dpWeights(:)
+  integer :: nExtIn, nExt, nGlobal, nAtomAll
+  real(dp), allocatable :: vals(:,:), atomic(:,:), sums(:,:), forces(:,:), gTargets(:,:), &
+      & aTargets(:,:), grad(:,:), params(:,:), atomicWeights(:), dpWeights(:), extRow(:)
   real(dp) :: loss, lr, maxDisp, tol, xyz(3)
   integer :: zz, sp
   logical :: tConverged
@@ -60,6 +64,9 @@ program fnet_fortran_check
     read(fd, *) model%zMean
     read(fd, *) model%zStd
   end if
+  read(fd, *) nExtIn
+  allocate(model%extIn(nExtIn))
+  if (nExtIn > 0) read(fd, *) model%extIn
   read(fd, *) nS
   allocate(model%speciesZ(nS), model%activation(nS))
   read(fd, *) model%speciesZ
@@ -72,24 +79,30 @@ program fnet_fortran_check
   read(fd, *) model%params
   nOut = model%dims(nL)
   read(fd, *) nS
-  allocate(geos(nS))
+  read(fd, *) nExt
+  allocate(geos(nS), extRow(nExt))
   do iS = 1, nS
     read(fd, *) nA, per
     geos(iS)%tPeriodic = per /= 0
     read(fd, *) geos(iS)%latVecs
     allocate(geos(iS)%coords(3, nA), geos(iS)%atomicNumbers(nA), geos(iS)%species(nA))
+    if (nExt > 0) allocate(geos(iS)%ext(nExt, nA))
     do iA = 1, nA
-      read(fd, *) xyz, zz, sp
+      read(fd, *) xyz, zz, sp, extRow
       geos(iS)%coords(:, iA) = xyz
       geos(iS)%atomicNumbers(iA) = zz
       geos(iS)%species(iA) = sp
+      if (nExt > 0) geos(iS)%ext(:, iA) = extRow
     end do
   end do
+  nAtomAll = sum([(size(geos(iS)%coords, dim=2), iS = 1, nS)])
   read(fd, *) hasTrain
   if (hasTrain /= 0) then
-    allocate(targets(nOut, nS), dpWeights(nS))
-    read(fd, *) targets
-    allocate(atomicWeights(sum([(size(geos(iS)%coords, dim=2), iS = 1, nS)])))
+    read(fd, *) nGlobal
+    allocate(gTargets(nGlobal, nS), aTargets(nOut - nGlobal, nAtomAll), dpWeights(nS), &
+        & atomicWeights(nAtomAll))
+    if (nGlobal > 0) read(fd, *) gTargets
+    if (nGlobal < nOut) read(fd, *) aTargets
     read(fd, *) atomicWeights
     read(fd, *) dpWeights
     read(fd, *) lr, maxDisp, tol
@@ -109,7 +122,7 @@ program fnet_fortran_check
   call writeBlock(fo, 'A', atomic)
   call writeBlock(fo, 'F', forces)
   if (hasTrain /= 0) then
-    call gpu%trainGradients(targets, atomicWeights, dpWeights, grad, loss)
+    call gpu%trainGradients(nGlobal, gTargets, aTargets, atomicWeights, dpWeights, grad, loss)
     call gpu%steepestDescent(lr, maxDisp, tol, params, tConverged)
     write(fo, '(a)') 'LOSS 1 1'
     write(fo, '(es26.17e3)') loss

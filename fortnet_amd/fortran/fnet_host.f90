@@ -8,12 +8,16 @@
 !>                               calculatePrime: forces(3 * nOut, nAtom), energy unit per Bohr;
 !>  * TFnetGpu%trainGradients -- updateGradients + sysTrain + the normalisation of TBpnn_update
 !>                               (bpnn.F90:394-481,610-704,751-767): FP64 gradient (nParams,
-!>                               nSpecies) of the mse loss, and the loss;
+!>                               nSpecies) of the mse loss over global targets (the first
+!>                               nGlobal outputs summed per structure) and atomic targets (the
+!>                               other outputs per atom), and the loss;
 !>  * TFnetGpu%steepestDescent -- SteepDesc_next (steepdesc.F90:125-204) on that gradient.
 !>
 !> Layouts are the reference's: coordinates (3, nAtom) in Bohr, lattice vectors as the columns
 !> of latVecs(3, 3), 1-based species (network) indices, parameters (nParams, nSpecies) in the
-!> serialised order of network.F90:397-427.  A failing call stops the program with the library's
+!> serialised order of network.F90:397-427, external atomic features ext(nExt, nAtom) of each
+!> structure (fnetdata.F90:1141-1162) and the 1-based indices of the ones the networks read
+!> after the ACSF (TExtFeaturesConfig, netstat.F90:163-211).  A failing call stops the program with the library's
 !> message, as the reference's error() does (message.F90:73-84).
 module fnet_host
   use, intrinsic :: iso_c_binding
@@ -39,6 +43,9 @@ module fnet_host
     integer, allocatable :: dims(:), activation(:)
     !> (nParams, nSpecies)
     real(dp), allocatable :: params(:,:)
+    !> 1-based indices of the external atomic features fed to the networks after the ACSF
+    !> (unallocated or empty: none)
+    integer, allocatable :: extIn(:)
   end type TFnetModel
 
   !> one structure (TGeometry, typegeometry.F90:24-59)
@@ -49,6 +56,9 @@ module fnet_host
     integer, allocatable :: atomicNumbers(:)
     !> 1-based network index of each atom
     integer, allocatable :: species(:)
+    !> external atomic features (nExt, nAtom); unallocated: none (every structure of a batch
+    !> has the same nExt)
+    real(dp), allocatable :: ext(:,:)
   end type TFnetGeometry
 
   !> one device context, its model and one resident batch of structures
@@ -125,6 +135,7 @@ contains
         & act(:)
     real(dp), allocatable, target :: rCut(:), eta(:), rs(:), kappa(:), lambda(:), xi(:), &
         & zMean(:), zStd(:), params(:,:)
+    integer(c_int), allocatable, target :: extIdx(:)
     integer(c_int), target :: noExt(1)
 
     this%nG = model%nG
@@ -173,6 +184,13 @@ contains
     end if
     md%n_ext_in = 0
     md%ext_in_idx = c_loc(noExt)
+    if (allocated(model%extIn)) then
+      if (size(model%extIn) > 0) then
+        extIdx = model%extIn
+        md%n_ext_in = int(size(extIdx), c_int)
+        md%ext_in_idx = c_loc(extIdx)
+      end if
+    end if
     md%n_species = int(this%nSpecies, c_int)
     md%species_z = c_loc(spz)
     md%n_layers = int(size(model%dims), c_int)
@@ -192,7 +210,8 @@ contains
     integer(c_long_long), allocatable, target :: offsets(:)
     real(dp), allocatable, target :: coords(:,:), latVecs(:,:,:)
     integer(c_int), allocatable, target :: periodic(:), species(:), atnum(:)
-    integer :: iS, nA, iA
+    real(dp), allocatable, target :: ext(:,:)
+    integer :: iS, nA, iA, nExt
 
     if (c_associated(this%batch)) call fnet_batch_destroy(this%batch)
     this%batch = c_null_ptr
@@ -207,6 +226,27 @@ contains
     offsets(:) = int(this%iStart - 1, c_long_long)
     allocate(coords(3, this%nAtom), latVecs(3, 3, this%nStruct), periodic(this%nStruct), &
         & species(this%nAtom), atnum(this%nAtom))
+    nExt = 0
+    if (this%nStruct > 0) then
+      if (allocated(geos(1)%ext)) nExt = size(geos(1)%ext, dim=1)
+    end if
+    allocate(ext(nExt, this%nAtom))
+    do iS = 1, this%nStruct
+      iA = this%iStart(iS)
+      nA = size(geos(iS)%coords, dim=2)
+      if (nExt == 0) then
+        if (allocated(geos(iS)%ext)) then
+          if (size(geos(iS)%ext, dim=1) /= 0) call stopWith('structures with and without '&
+              & // 'external atomic features in one batch')
+        end if
+        cycle
+      end if
+      if (.not. allocated(geos(iS)%ext)) call stopWith('structures with and without external '&
+          & // 'atomic features in one batch')
+      if (size(geos(iS)%ext, dim=1) /= nExt .or. size(geos(iS)%ext, dim=2) /= nA) &
+          & call stopWith('external atomic features of inconsistent shape')
+      ext(:, iA:iA + nA - 1) = geos(iS)%ext
+    end do
     do iS = 1, this%nStruct
       iA = this%iStart(iS)
       nA = size(geos(iS)%coords, dim=2)
@@ -223,8 +263,9 @@ contains
     bd%periodic = c_loc(periodic)
     bd%species = c_loc(species)
     bd%atnum = c_loc(atnum)
-    bd%n_ext = 0
+    bd%n_ext = int(nExt, c_int)
     bd%ext = c_null_ptr
+    if (nExt > 0) bd%ext = c_loc(ext)
     call check(this, fnet_batch_upload(this%ctx, bd, this%batch), 'fnet_batch_upload')
     call this%setPrecision(precision)
   end subroutine TFnetGpu_setGeometries
@@ -271,24 +312,43 @@ contains
   end subroutine TFnetGpu_forceAnalysis
 
 
-  !> mse gradient of the network parameters against global targets (nOut, nStruct), with the
-  !> atomic weights (nAtom) and datapoint weights (nStruct), normalised as TBpnn_update;
-  !> returns it and the loss
-  subroutine TFnetGpu_trainGradients(this, targets, atomicWeights, dpWeights, grad, loss)
+  !> mse gradient of the network parameters, normalised as TBpnn_update, and the loss: the first
+  !> nGlobal outputs are summed per structure against globalTargets (nGlobal, nStruct), the
+  !> other nOut - nGlobal outputs are compared per atom with atomicTargets (nOut - nGlobal,
+  !> nAtom); atomic weights (nAtom) and datapoint weights (nStruct) as in the dataset
+  subroutine TFnetGpu_trainGradients(this, nGlobal, globalTargets, atomicTargets, &
+      & atomicWeights, dpWeights, grad, loss)
     class(TFnetGpu), intent(inout) :: this
-    real(dp), intent(in) :: targets(:,:), atomicWeights(:), dpWeights(:)
+    integer, intent(in) :: nGlobal
+    real(dp), intent(in) :: globalTargets(:,:), atomicTargets(:,:), atomicWeights(:), &
+        & dpWeights(:)
     real(dp), intent(out), allocatable, target :: grad(:,:)
     real(dp), intent(out), target :: loss
-    real(dp), allocatable, target :: tgt(:,:), aw(:), dw(:)
+    real(dp), allocatable, target :: gt(:,:), at(:,:), aw(:), dw(:)
+    type(c_ptr) :: pGt, pAt
 
-    tgt = targets
+    if (nGlobal < 0 .or. nGlobal > this%nOut) call stopWith('invalid number of global targets')
+    if (nGlobal > 0) then
+      if (size(globalTargets, dim=1) /= nGlobal .or. size(globalTargets, dim=2) /= this%nStruct) &
+          & call stopWith('global targets must be (nGlobal, nStruct)')
+    end if
+    if (nGlobal < this%nOut) then
+      if (size(atomicTargets, dim=1) /= this%nOut - nGlobal &
+          & .or. size(atomicTargets, dim=2) /= this%nAtom) &
+          & call stopWith('atomic targets must be (nOut - nGlobal, nAtom)')
+    end if
+    gt = globalTargets
+    at = atomicTargets
     aw = atomicWeights
     dw = dpWeights
+    pGt = c_null_ptr
+    pAt = c_null_ptr
+    if (nGlobal > 0) pGt = c_loc(gt)
+    if (nGlobal < this%nOut) pAt = c_loc(at)
     allocate(grad(this%nParams, this%nSpecies))
     call check(this, fnet_set_loss(this%ctx, FNET_LOSS_MSE), 'fnet_set_loss')
-    call check(this, fnet_train_gradients(this%ctx, this%batch, int(this%nOut, c_int), &
-        & c_loc(tgt), c_null_ptr, c_loc(aw), c_loc(dw), c_loc(grad), c_null_ptr, c_loc(loss)), &
-        & 'fnet_train_gradients')
+    call check(this, fnet_train_gradients(this%ctx, this%batch, int(nGlobal, c_int), pGt, pAt, &
+        & c_loc(aw), c_loc(dw), c_loc(grad), c_null_ptr, c_loc(loss)), 'fnet_train_gradients')
   end subroutine TFnetGpu_trainGradients
 
 
@@ -305,6 +365,15 @@ contains
         & c_loc(params), conv), 'fnet_optimizer_step_sd')
     tConverged = conv /= 0
   end subroutine TFnetGpu_steepestDescent
+
+
+  !> stops with a message, as the reference's error() does
+  subroutine stopWith(msg)
+    character(*), intent(in) :: msg
+
+    write(error_unit, '(a)') 'ERROR! ' // msg
+    error stop 1
+  end subroutine stopWith
 
 
   subroutine TFnetGpu_destroy(this)

@@ -38,23 +38,33 @@ def write_case(path, ns, geos, precision, train=None):
         L += ["1", " ".join("%.17g" % x for x in ns.zmean), " ".join("%.17g" % x for x in ns.zstd)]
     else:
         L.append("0")
+    ext_in = [int(k) for k in (ns.ext_in if ns.ext_in is not None else [])]
+    L.append(str(len(ext_in)))
+    if ext_in:
+        L.append(" ".join(str(k) for k in ext_in))
     L += [str(len(ns.species_z)), " ".join(str(int(z)) for z in ns.species_z),
           " ".join(str(int(a)) for a in ns.activation), str(len(ns.dims)),
           " ".join(str(int(d)) for d in ns.dims), str(ns.params.shape[1])]
     L += ["%.17g" % x for x in ns.params.reshape(-1)]
     L.append(str(len(geos)))
+    next_ = geos[0].ext.shape[1] if geos else 0
+    L.append(str(next_))
     for geo in geos:
         L.append("%d %d" % (geo.n, 1 if geo.periodic else 0))
         L.append(" ".join("%.17g" % x for x in geo.latvecs.reshape(-1)))
         for a in range(geo.n):
-            L.append("%.17g %.17g %.17g %d %d" % (tuple(geo.coords[a]) + (geo.atnum[a],
-                                                                          geo.species[a] + 1)))
+            L.append(" ".join(["%.17g %.17g %.17g %d %d" % (tuple(geo.coords[a]) + (
+                geo.atnum[a], geo.species[a] + 1))] + ["%.17g" % x for x in geo.ext[a]]))
     if train is None:
         L.append("0")
     else:
-        targets, aw, dw, lr, maxdisp, tol = train
+        nglob, gtargets, atargets, aw, dw, lr, maxdisp, tol = train
         L.append("1")
-        L += ["%.17g" % x for x in np.asarray(targets, np.float64).reshape(-1)]
+        L.append(str(int(nglob)))
+        if nglob > 0:  # (nGlobal, nStruct) in Fortran order = [nStruct][nGlobal] rows
+            L += ["%.17g" % x for x in np.asarray(gtargets, np.float64).reshape(-1)]
+        if nglob < ns.n_out:  # (nOut - nGlobal, nAtom) in Fortran order = [nAtom][...] rows
+            L += ["%.17g" % x for x in np.asarray(atargets, np.float64).reshape(-1)]
         L += ["%.17g" % x for x in aw]
         L += ["%.17g" % x for x in dw]
         L.append("%.17g %.17g %.17g" % (lr, maxdisp, tol))
@@ -102,7 +112,11 @@ def test_driver_stops_with_message_without_device(tmp_path):
 
 FORCE_CASES = [n for n in case_names() if n.startswith("analysis__derivatives__analytical")
                and "atomID" not in n]
-SD_CASES = [n for n in case_names("restart") if n.endswith("steepestdesc")]
+# every steepest-descent restart: global, atomic and both kinds of targets
+SD_CASES = [n for n in case_names("restart") if str(load_case(n)[0]["optimizer"]) == "sd"]
+# predictions with external atomic features beside the ACSF (mixed) or an empty external list
+EXT_CASES = ["input__mixedfeatures__prediction", "input__mixedfeatures__validation",
+             "input__extfeatures__prediction__empty", "input__extfeatures__validation__empty"]
 
 
 @needs_driver
@@ -131,13 +145,13 @@ def test_fortran_host_sd_step_matches_fixture(name, tmp_path):
     the netstat the reference wrote (sysTrain/bprop and steepdesc.F90:160-204)."""
     c, ds = load_case(name)
     ns = fa.Netstat.from_arrays(c, "in_")
-    if str(c["optimizer"]) != "sd" or ns.n_global != ns.n_out or len(ns.ext_in):
-        pytest.skip("not a steepest-descent case with global targets only")
     print("case", name, flush=True)
     geos = dataset.geometries(ds)
     maxdisp = float(c["maxdisp"]) if float(c["maxdisp"]) > 0 else 1e4
-    train = (ds["globaltargets"], ds["atomicweights"], ds["weights"].astype(np.float64),
-             float(c["learningrate"]), maxdisp, 1e-20)
+    ng = ns.n_global
+    train = (ng, ds["globaltargets"] if ng > 0 else None,
+             ds["atomictargets"] if ng < ns.n_out else None, ds["atomicweights"],
+             ds["weights"].astype(np.float64), float(c["learningrate"]), maxdisp, 1e-20)
     write_case(tmp_path / "case.txt", ns, geos, fa.FNET_FP64, train)
     r = run_driver(tmp_path / "case.txt", tmp_path / "out.txt")
     assert r.returncode == 0, r.stderr
@@ -145,3 +159,23 @@ def test_fortran_host_sd_step_matches_fixture(name, tmp_path):
     new = res["PARAMS"].T  # [nSpecies, nParams]
     assert np.allclose(new, c["after_params"], atol=ATOL, rtol=RTOL), \
         np.abs(new - c["after_params"]).max()
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", EXT_CASES)
+def test_fortran_host_external_features(name, tmp_path):
+    """External atomic features through the Fortran types (TFnetGeometry%ext, TFnetModel%extIn):
+    the raw predictions of the reference's external-feature runs, element-wise."""
+    c, ds = load_case(name)
+    print("case", name, flush=True)
+    ns = fa.Netstat.from_arrays(c, "in_")
+    geos = dataset.geometries(ds)
+    write_case(tmp_path / "case.txt", ns, geos, fa.FNET_FP64)
+    r = run_driver(tmp_path / "case.txt", tmp_path / "out.txt")
+    assert r.returncode == 0, r.stderr
+    res = read_result(tmp_path / "out.txt")
+    got = res["A"].T
+    assert np.allclose(got, c["out_raw"], atol=ATOL, rtol=RTOL), np.abs(got - c["out_raw"]).max()
+    if "out_glob" in c and c["out_glob"].size:
+        assert np.allclose(res["E"].T[:, :ns.n_global], c["out_glob"], atol=ATOL, rtol=RTOL)

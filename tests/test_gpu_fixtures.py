@@ -20,7 +20,13 @@ pytestmark = pytest.mark.gpu
 
 ATOL, RTOL = 1e-10, 1e-9  # testwithworkdir.py:24-25
 
+# with the prediction / validation runs of the input tests with external atomic features (ACSF
+# only, external features only, both mixed)
+EXT_CASES = [n for n in case_names("input__") if n.startswith((
+    "input__extfeatures__prediction", "input__extfeatures__validation",
+    "input__mixedfeatures__prediction", "input__mixedfeatures__validation"))]
 PRED_CASES = [n for n in case_names() if n.startswith(("analysis", "prediction", "validation"))]
+PRED_CASES += EXT_CASES
 SD_CASES = [n for n in case_names("restart") if n.endswith("steepestdesc") or "acsffeatures" in n]
 
 

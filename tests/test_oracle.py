@@ -27,7 +27,13 @@ def _close(a, b):
     return np.allclose(a, b, atol=ATOL, rtol=RTOL)
 
 
-PRED = [n for n in case_names() if n.startswith(("analysis", "prediction", "validation"))]
+# prediction / validation runs of the reference's input-parsing tests with external atomic
+# features: ACSF only ("empty" external list), external features only, and both mixed
+EXT = [n for n in case_names("input__") if n.startswith(("input__extfeatures__prediction",
+                                                           "input__extfeatures__validation",
+                                                           "input__mixedfeatures__prediction",
+                                                           "input__mixedfeatures__validation"))]
+PRED = [n for n in case_names() if n.startswith(("analysis", "prediction", "validation"))] + EXT
 
 
 @pytest.mark.parametrize("name", PRED)
