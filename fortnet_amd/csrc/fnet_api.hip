@@ -716,6 +716,15 @@ int check_ready(fnet_ctx* ctx, fnet_batch* b) {
   if (b->ctx != ctx) return fail(ctx, FNET_ERR_ARG, "batch belongs to another context");
   if (!ctx->model.valid) return fail(ctx, FNET_ERR_STATE, "no model uploaded");
   HIPCHECK(ctx, hipSetDevice(ctx->device));
+  // results of an earlier model (list cutoff, functions, networks) or of earlier parameters
+  // are stale
+  if (b->seen_model_gen != ctx->model_gen) {
+    invalidate(b);
+    b->seen_model_gen = ctx->model_gen;
+  } else if (b->seen_params_gen != ctx->params_gen) {
+    b->have_E = b->have_F = false;
+  }
+  b->seen_params_gen = ctx->params_gen;
   return ensure_results(b);
 }
 
@@ -979,6 +988,7 @@ int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* md) {
   free_model(ctx->model);
   ctx->model = m;
   ctx->model_gen++;
+  ctx->params_gen++;
   ctx->have_grad = false;
   return FNET_OK;
 }
@@ -995,6 +1005,7 @@ int fnet_params_upload(fnet_ctx* ctx, const double* params) {
                                ctx->stream));
   HIPCHECK(ctx, launch_to_float(m.params, m.params_f, (long long)np, ctx->stream));
   HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->params_gen++;
   return upload_wpack(ctx, m, ctx->stream);
 }
 
@@ -1570,6 +1581,7 @@ int fnet_optimizer_step_sd(fnet_ctx* ctx, double lr, double max_disp, double tol
   double* gmax = ctx->d_grad + np;
   HIPCHECK(ctx, launch_sd_step(m.params, ctx->d_grad, m.shape.nparams, lr, max_disp, tol, gmax,
                                m.nspecies, ctx->stream));
+  ctx->params_gen++;
   HIPCHECK(ctx, launch_to_float(m.params, m.params_f, np, ctx->stream));
   std::vector<double> gm(m.nspecies);
   HIPCHECK(ctx, hipMemcpyAsync(m.h_params.data(), m.params, sizeof(double) * np,
@@ -1657,6 +1669,7 @@ int fnet_optimizer_step(fnet_ctx* ctx, int kind, double* params_out, int* conver
   const long long n = m.shape.nparams, np = n * m.nspecies;
   HIPCHECK(ctx, launch_opt_step(ctx->opt_states, m.params, ctx->d_grad, ctx->opt_vecs, n,
                                 ctx->opt_vstride, ctx->last_loss, ctx->opt_conv, m.nspecies, s));
+  ctx->params_gen++;
   HIPCHECK(ctx, launch_to_float(m.params, m.params_f, np, s));
   std::vector<int> conv(m.nspecies, 0);
   HIPCHECK(ctx, hipMemcpyAsync(m.h_params.data(), m.params, sizeof(double) * np,

@@ -284,10 +284,11 @@ extern "C" int fnet_forces_noalloc(fnet_ctx* ctx, fnet_batch* b) {
   if (!b || b->ctx != ctx) return noalloc_fail(ctx, FNET_ERR_ARG, "batch of another context");
   const DevModel& m = ctx->model;
   if (!m.valid) return noalloc_fail(ctx, FNET_ERR_STATE, "no model uploaded");
-  if (!b->have_F || !b->have_nl || b->result_precision != b->precision)
+  if (!b->have_F || !b->have_nl || b->result_precision != b->precision ||
+      b->seen_model_gen != ctx->model_gen || b->seen_params_gen != ctx->params_gen)
     return noalloc_fail(ctx, FNET_ERR_STATE,
                         "fnet_forces_noalloc needs a batch evaluated with forces in its current "
-                        "mode (fnet_run or fnet_forces first)");
+                        "mode and with the current parameters (fnet_run or fnet_forces first)");
   if (hipSetDevice(ctx->device) != hipSuccess)
     return noalloc_fail(ctx, FNET_ERR_HIP, "hipSetDevice failed");
   unsigned char* base = (unsigned char*)scratch_ptr(ctx->device);

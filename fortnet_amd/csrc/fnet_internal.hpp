@@ -163,6 +163,7 @@ struct fnet_ctx {
   int* opt_conv = nullptr;               // [nspecies]
   long long opt_vstride = 0, opt_cap = 0;
   long long model_gen = 0;    // bumped by every model upload (new device tables)
+  long long params_gen = 0;   // bumped by every change of the parameters (uploads, optimizer steps)
   double reg_strength = 0.0;  // elastic-net regularisation of the weight gradients
   double reg_alpha = 0.0;
 };
@@ -278,6 +279,8 @@ struct fnet_batch {
   double* d_targets = nullptr; // staging of training inputs
   long long targets_cap = 0;
   bool have_nl = false, have_G = false, have_E = false, have_F = false;
+  // model and parameter generations the batch's results were computed with
+  long long seen_model_gen = -1, seen_params_gen = -1;
   int result_precision = -1;
 };
 

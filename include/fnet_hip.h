@@ -102,10 +102,13 @@ int fnet_last_error(const fnet_ctx* ctx, char* buf, int len);
 long long fnet_params_per_species(int n_layers, const int* dims); /* network.F90:126-142 */
 
 /* Uploads the ACSF table, z-score and the per-species networks, resident until replaced.
- * Replaces TAcsf%fromFile (acsf.F90:1908-2135) + TBpnn%fromFile (bpnn.F90:1141-1288). */
+ * Replaces TAcsf%fromFile (acsf.F90:1908-2135) + TBpnn%fromFile (bpnn.F90:1141-1288).  Every
+ * batch of the context recomputes its results (neighbour list included) when next read. */
 int fnet_model_upload(fnet_ctx* ctx, const fnet_model_desc* model);
 
-/* Replaces only the network parameters (after an optimizer step), same layout as params. */
+/* Replaces only the network parameters (after an optimizer step), same layout as params.  The
+ * batches keep their neighbour lists and features and recompute outputs and forces when next
+ * read (as after fnet_optimizer_step / fnet_optimizer_step_sd). */
 int fnet_params_upload(fnet_ctx* ctx, const double* params);
 
 /* Uploads a batch of structures; the batch stays resident and owns its neighbour state.
@@ -218,7 +221,8 @@ int fnet_optimizer_step(fnet_ctx* ctx, int kind, double* params_out, int* conver
 
 /* The force assembly of fnet_forces (adjoint pass, mirror entries of the neighbour list and the
  * per-atom gather) for the batch's resident neighbour list and network input gradients (the
- * batch must have been evaluated with forces in its current mode: fnet_run or fnet_forces),
+ * batch must have been evaluated with forces in its current mode and with the current
+ * parameters: fnet_run or fnet_forces; otherwise FNET_ERR_STATE),
  * writing the batch's force buffer (download with fnet_forces).  No counterpart in the
  * reference: it is for callers that cannot allocate -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  It allocates nothing, does not synchronise

@@ -145,3 +145,35 @@ def test_update_coords_rejects_bad_cells(ctx):
     b.update_coords(coords, lats)
     assert np.array_equal(b.forces(), F_ref)
     b.close()
+
+
+def test_results_follow_parameter_and_model_uploads(ctx):
+    """A batch evaluated before new parameters (or a new model) are uploaded recomputes its
+    outputs and forces when they are next read: they equal those of a fresh batch."""
+    geos, ns = _zscored(ctx, "water", 0.004)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    E0, _ = b.energy()
+    b.forces()
+    p2 = ns.params * 1.1
+    ctx.upload_params(p2)
+    E1, Ea1 = b.energy()
+    F1 = b.forces()
+    fresh = ctx.batch(geos, fa.FNET_FP64)
+    E2, Ea2 = fresh.energy()
+    F2 = fresh.forces()
+    fresh.close()
+    assert not np.allclose(E1, E0)
+    assert np.array_equal(Ea1, Ea2) and np.array_equal(F1, F2)
+    # a model with other functions (same counts) changes the features too
+    g = ns.g
+    g2 = fa.GFunctions(g.type, g.rcut * 0.9, eta=g.eta, rs=g.rs, kappa=g.kappa, lam=g.lam,
+                       xi=g.xi, atomid=g.atomid, z1=g.z1, z2=g.z2)
+    ns2 = fa.Netstat(g2, ns.species_z, ns.dims, ns.activation, p2, ns.zmean, ns.zstd)
+    ctx.upload_model(ns2)
+    G3 = b.acsf()
+    E3, _ = b.energy()
+    fresh = ctx.batch(geos, fa.FNET_FP64)
+    assert np.array_equal(G3, fresh.acsf())
+    assert np.array_equal(E3, fresh.energy()[0])
+    fresh.close()
+    b.close()

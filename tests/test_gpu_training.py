@@ -117,3 +117,60 @@ def test_training_restart(ctx, name, device_opt):
     ns = Netstat.from_arrays(c, "in_")
     res = _run(ctx, c, ds, ns, None, device_opt)
     _close(res.params, c["after_params"], "parameters after one iteration")
+
+
+@pytest.mark.parametrize("kind", ["mse", "mae", "mape", "rms"])
+def test_host_batch_loss_equals_device_loss(ctx, kind):
+    """The validation monitor's host loss (batch_loss) equals the device loss of
+    fnet_train_gradients on the same predictions, targets and weights."""
+    from fortnet_amd.train import batch_loss
+    c, ds = load_case("restart__multispecies__bothTargets")  # global and atomic targets
+    ns = Netstat.from_arrays(c, "in_")
+    ctx.upload_model(ns)
+    ctx.set_loss(kind)
+    geos = dataset.geometries(ds)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    gt, at = _targets(ds, ns)
+    w = ds["weights"].astype(np.float64)
+    b.train_gradients(gt, at, ds["atomicweights"], w)
+    Es, Ea = b.energy()
+    host = batch_loss(Es, Ea, ds["offsets"], ns.n_global, gt, at, ds["atomicweights"], w, kind)
+    assert abs(host - b.loss) <= 1e-12 * abs(b.loss)
+    b.close()
+    ctx.set_loss("mse")
+
+
+def test_validation_monitor_checkpoints_and_progress(ctx, tmp_path):
+    """Five L-BFGS iterations with the training set as validation set (unit datapoint weights,
+    so the reference's unweighted validation loss equals the training loss after every
+    update), a progress line every 2 iterations and a netstat every 2: the validation losses
+    follow the training losses, the lines come at iterations 2, 4, 5, and the netstat written
+    at iteration 2 / at the end holds the parameters of a 2-iteration / the full run."""
+    from fortnet_amd import h5io
+    name = [n for n in SCRATCH if "lbfgs" in n][0]
+    c, ds = load_case(name)
+    ref = Netstat.from_arrays(c, "after_")
+    geos = dataset.geometries(ds)
+    gt, at = _targets(ds, ref)
+    lines = []
+    path = str(tmp_path / "fortnet.hdf5")
+    kw = dict(optimizer="lbfgs", learning_rate=float(c["learningrate"]), tol=1e-20,
+              min_disp=float(c["mindisp"]),
+              max_disp=float(c["maxdisp"]) if float(c["maxdisp"]) > 0 else 1e4,
+              memory=int(c["memory"]), line_search=bool(int(c["linemin"])))
+    start = Netstat(ref.g, ref.species_z, ref.dims, ref.activation, c["after_params"],
+                    ref.zmean, ref.zstd, ref.ext_in, ref.n_global)
+    w1 = np.ones(len(geos))
+    res = train(ctx, start, geos, gt, at, ds["atomicweights"], w1, n_iter=5, valid_geos=geos,
+                valid_g_targets=gt, valid_a_targets=at, valid_atomic_weights=ds["atomicweights"],
+                n_print=2, n_save=2, netstat_path=path, out=lines.append, **kw)
+    print("losses", res.losses, "valid", res.valid_losses)
+    assert len(res.valid_losses) == res.iterations == 5
+    assert np.allclose(res.valid_losses, res.train_losses, rtol=1e-12, atol=0)
+    assert [int(l.split()[0]) for l in lines] == [2, 4, 5]
+    assert all(len(l.split()) == 4 for l in lines)
+    final = h5io.load_netstat(path)
+    assert np.array_equal(final.params, res.params)
+    res2 = train(ctx, start, geos, gt, at, ds["atomicweights"], w1, n_iter=2, n_save=2,
+                 netstat_path=str(tmp_path / "two.hdf5"), **kw)
+    assert np.array_equal(h5io.load_netstat(str(tmp_path / "two.hdf5")).params, res2.params)
