@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--train-steps", type=int, default=3)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch parameters (no autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
@@ -350,6 +352,35 @@ def main():
             if m["within_gates"] and (best is None or t < best[2]):
                 best = (name, how, t, st)
     name, how, dt, stage = best
+    # launch parameters: the defaults above; then the autotuner's choice for the headline mode
+    # and, for comparison, the same with persistent ACSF grids forced on
+    launch_params = {"default": {"eager_ms_per_step": modes[name]["eager"]["ms_per_step"]}}
+    if "graph" in modes[name]:
+        launch_params["default"]["graph_ms_per_step"] = modes[name]["graph"]["ms_per_step"]
+    if not args.no_tune:
+        b.set_mode(MODES[name], fa.FNET_BATCHED)
+        params, t_def, t_tun = b.autotune(reps=3)
+        tuned = {"params": params, "tuner_step_ms_default": t_def, "tuner_step_ms_tuned": t_tun}
+        for g in ([False] if args.no_graph else [False, True]):
+            dtt, stt, _, rep = time_steps(ctx, b, args.steps, args.warmup, g)
+            tuned[("graph" if g else "eager") + "_ms_per_step"] = 1e3 * dtt / args.steps
+            if g and not rep:
+                continue
+            if dtt < dt:
+                how, dt = ("tuned_graph" if g else "tuned_eager"), dtt
+                if not g:
+                    stage = stt
+        launch_params["tuned"] = tuned
+        if not params["persist"]:
+            b.set_tune(**dict(params, persist=1))
+            dtp, _, _, _ = time_steps(ctx, b, args.steps, args.warmup, False)
+            launch_params["tuned_persistent_eager_ms_per_step"] = 1e3 * dtp / args.steps
+            b.set_tune(**params)
+        else:
+            b.set_tune(**dict(params, persist=0))
+            dtp, _, _, _ = time_steps(ctx, b, args.steps, args.warmup, False)
+            launch_params["tuned_nonpersistent_eager_ms_per_step"] = 1e3 * dtp / args.steps
+            b.set_tune(**params)
     value = args.steps * N / dt
     ms_per_step = 1e3 * dt / args.steps
 
@@ -412,6 +443,7 @@ def main():
         "plain_fp64_per_structure_atoms_per_s": plain,
         "batched_over_plain": value / plain if plain else None,
         "precision_modes_vs_fp64": modes,
+        "launch_parameters": launch_params,
         "stages_ms": dict(zip(names, [float(x) for x in stage])),
         "plain_stages_ms": dict(zip(names, [float(x) for x in stagep])),
         "train_gradients": {"precision_mode": name, "ms": 1e3 * t_train,

@@ -30,7 +30,7 @@ EXPORTS = [
     "fnet_stage_times", "fnet_batch_reload", "fnet_query", "fnet_set_loss",
     "fnet_set_regularization", "fnet_set_graph", "fnet_batch_sync", "fnet_step_time",
     "fnet_batch_step_info", "fnet_optimizer_init", "fnet_optimizer_step", "fnet_device_memory",
-    "fnet_forces_noalloc",
+    "fnet_forces_noalloc", "fnet_set_tune", "fnet_get_tune", "fnet_autotune",
 ]
 
 
@@ -104,6 +104,9 @@ def load():
         "fnet_optimizer_step": (i32, [vp, i32, vp, ctypes.POINTER(i32)]),
         "fnet_device_memory": (i32, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "fnet_forces_noalloc": (i32, [vp, vp]),
+        "fnet_set_tune": (i32, [vp, vp, vp]),
+        "fnet_get_tune": (i32, [vp, vp, vp]),
+        "fnet_autotune": (i32, [vp, vp, i32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -433,6 +433,27 @@ class Batch:
     def run(self):
         check(self.ctx.h, self.lib.fnet_run(self.ctx.h, self.h), "fnet_run")
 
+    TUNE_KNOBS = ("fwd_wpb", "adj_wpb", "mlp_waves", "rbal", "persist")
+
+    def set_tune(self, fwd_wpb=0, adj_wpb=0, mlp_waves=0, rbal=-1, persist=0):
+        """Launch parameters of this batch's model, precision and size (``fnet_set_tune``)."""
+        k = np.array([fwd_wpb, adj_wpb, mlp_waves, rbal, persist], np.int32)
+        check(self.ctx.h, self.lib.fnet_set_tune(self.ctx.h, self.h, ptr(k)), "fnet_set_tune")
+
+    def get_tune(self):
+        k = np.zeros(5, np.int32)
+        check(self.ctx.h, self.lib.fnet_get_tune(self.ctx.h, self.h, ptr(k)), "fnet_get_tune")
+        return dict(zip(self.TUNE_KNOBS, (int(x) for x in k)))
+
+    def autotune(self, reps=3):
+        """``fnet_autotune``: chooses the launch parameters by timing the batch's steady-state
+        step; returns (parameters, step ms with the defaults, step ms tuned)."""
+        ms = np.zeros(2)
+        k = np.zeros(5, np.int32)
+        check(self.ctx.h, self.lib.fnet_autotune(self.ctx.h, self.h, int(reps), ptr(ms), ptr(k)),
+              "fnet_autotune")
+        return dict(zip(self.TUNE_KNOBS, (int(x) for x in k))), float(ms[0]), float(ms[1])
+
     def sync(self):
         """Waits for the batch's last step and validates it (``fnet_batch_sync``)."""
         check(self.ctx.h, self.lib.fnet_batch_sync(self.h), "fnet_batch_sync")

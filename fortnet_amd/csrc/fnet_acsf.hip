@@ -159,6 +159,7 @@ struct AcsfArgs {
   double rclist;
   long long a0, a1;
   int maxnb;
+  int persist;      // unused by the kernels (the persistent grid is an instance of its own)
   double* G;        // [N][nG] (forward)
   const double* cG; // [N][nout][nG] (adjoint)
   double* F;        // [N][nout][3] (adjoint): the centre's own sum of its entries' gradients
@@ -1086,13 +1087,11 @@ __device__ __forceinline__ double radial_sum(const AcsfArgs& a, const Slice<T>& 
 // that the kernels of models without pairs keep their register budget).  R64 (FP32 forward
 // only): radial terms in FP64 (radial_sum), also an instance of its own.
 template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS, bool R64 = false>
-__device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
+__device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i) {
   using V = typename Vec4<T>::type;
   extern __shared__ __align__(32) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long long i = a.a0 + (long long)blockIdx.x * (blockDim.x >> 6) + wave;
-  if (i >= a.a1) return;
   const int M = a.maxnb;
   Slice<T> sl = make_slice<T, ADJ>(smem + slice_bytes<T, ADJ>(M, PAIRS) * wave, M, PAIRS);
 
@@ -1512,6 +1511,26 @@ __device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
   }
 }
 
+// One wave per centre.  PERSIST: a grid of as many blocks as stay resident takes the centres in
+// strides of its wave count, restaging the slice for each (an instance of its own: the loop
+// around the body costs the one-centre kernels registers).
+template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS, bool R64 = false,
+          bool PERSIST = false>
+__device__ __forceinline__ void acsf_wave(const AcsfArgs& a) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  long long i = a.a0 + (long long)blockIdx.x * wpb + wave;
+  if constexpr (!PERSIST) {
+    if (i < a.a1) acsf_centre<T, ADJ, MQ, LAD, PAIRS, R64>(a, i);
+  } else {
+    const long long stride = (long long)gridDim.x * wpb;
+    for (; i < a.a1; i += stride) {
+      acsf_centre<T, ADJ, MQ, LAD, PAIRS, R64>(a, i);
+      wave_sync();
+    }
+  }
+}
+
 template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS>
 __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
   acsf_wave<T, ADJ, MQ, LAD, PAIRS>(a);
@@ -1525,10 +1544,18 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5)))
   acsf_wave<float, ADJ, 1, true, false>(a);
 }
 
+// persistent grid of the same (the loop needs more than 96 registers: 4 waves per SIMD)
+template <bool ADJ>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+    k_acsf_lad32_persist(AcsfArgs a) {
+  acsf_wave<float, ADJ, 1, true, false, false, true>(a);
+}
+
 // FP32 ladder adjoint with paired groups: 102 registers without spilling at 4 waves per SIMD
+template <bool PERSIST = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
     k_acsf_lad32_pairs(AcsfArgs a) {
-  acsf_wave<float, true, 1, true, true>(a);
+  acsf_wave<float, true, 1, true, true, false, PERSIST>(a);
 }
 
 // FP32 forward with the radial terms in FP64 (models with an ill-conditioned radial z-score)
@@ -1542,13 +1569,14 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
                      hipStream_t s) {
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
+  const Tune& tn = tune_of(b);
   const int M = slice_entries(b->max_nb);
   // paired ladder groups share one adjoint walk when the larger slice still fits a block
   const int pairs = (ADJ && LAD && m.has_pairs && slice_bytes<T, ADJ>(M, 1) <= 64 * 1024) ? 1 : 0;
   const size_t sb = slice_bytes<T, ADJ>(M, pairs);
-  // waves per block (water boxes: adjoint 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 /
-  // 2.25 ms with 2 / 4)
-  int wpb = ADJ ? 1 : 2;
+  // waves per block: the tuned value, else 1 (adjoint) / 2 (forward) (water boxes: adjoint
+  // 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 / 2.25 ms with 2 / 4)
+  int wpb = ADJ ? (tn.adj_wpb > 0 ? tn.adj_wpb : 1) : (tn.fwd_wpb > 0 ? tn.fwd_wpb : 2);
   while (wpb > 1 && sb * wpb > 64 * 1024) wpb >>= 1;
   if (sb * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
   AcsfArgs a;
@@ -1562,7 +1590,7 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.next = b->next;
   a.radial = m.radial;
   a.nR = m.nR;
-  a.rbal = m.radial_resolved;
+  a.rbal = tn.rbal >= 0 ? tn.rbal : m.radial_resolved;
   a.groups = m.groups;
   a.nAG = m.nAG;
   a.nZ = m.nZ;
@@ -1572,32 +1600,44 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.a0 = a0;
   a.a1 = a1;
   a.maxnb = M;
+  a.persist = tn.persist ? 1 : 0;
   a.G = (double*)b->G;
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;
   a.gvec = b->gvec;
   a.mir = b->mir;
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
+  const size_t lds = sb * wpb;
+  // persistent grids exist for the FP32 exponent-ladder kernels (the batched FP32 / BF16 path);
+  // the other kernels run one wave per centre
+  auto go = [&](void (*kern)(AcsfArgs), bool persistent) -> hipError_t {
+    unsigned g = grid;
+    if (persistent) {
+      // as many blocks as stay resident on the device at once
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * wpb, lds) !=
+          hipSuccess)
+        per_cu = 1;
+      const long long full = (long long)std::max(per_cu, 1) * std::max(b->ctx->n_cu, 1);
+      g = (unsigned)std::min<long long>(grid, full);
+    }
+    hipLaunchKernelGGL(kern, dim3(g), dim3(64 * wpb), lds, s, a);
+    return hipGetLastError();
+  };
+  const bool ps = a.persist != 0;
   if constexpr (ADJ && LAD) {
     if (pairs) {
       if constexpr (std::is_same<T, float>::value)
-        k_acsf_lad32_pairs<<<grid, 64 * wpb, sb * wpb, s>>>(a);
-      else
-        k_acsf<T, true, 1, true, true><<<grid, 64 * wpb, sb * wpb, s>>>(a);
-      return hipGetLastError();
+        return ps ? go(k_acsf_lad32_pairs<true>, true) : go(k_acsf_lad32_pairs<false>, false);
+      else return go(k_acsf<T, true, 1, true, true>, false);
     }
   }
   if constexpr (!ADJ && std::is_same<T, float>::value) {
-    if (m.radial64) {
-      k_acsf_fwd32_r64<MQ, LAD><<<grid, 64 * wpb, sb * wpb, s>>>(a);
-      return hipGetLastError();
-    }
+    if (m.radial64) return go(k_acsf_fwd32_r64<MQ, LAD>, false);
   }
   if constexpr (LAD && std::is_same<T, float>::value)
-    k_acsf_lad32<ADJ><<<grid, 64 * wpb, sb * wpb, s>>>(a);
-  else
-    k_acsf<T, ADJ, MQ, LAD, false><<<grid, 64 * wpb, sb * wpb, s>>>(a);
-  return hipGetLastError();
+    return ps ? go(k_acsf_lad32_persist<ADJ>, true) : go(k_acsf_lad32<ADJ>, false);
+  else return go(k_acsf<T, ADJ, MQ, LAD, false>, false);
 }
 
 template <typename T, bool ADJ>

@@ -12,6 +12,25 @@
 
 using namespace fnet;
 
+namespace fnet {
+
+// Autotuned launch parameters are kept per (model generation, precision, log2 of the atom
+// count): a batch of the same model and about the same size reuses them.
+std::vector<long long> tune_key(const fnet_batch* b) {
+  long long lg = 0;
+  while ((1LL << (lg + 1)) <= b->N) ++lg;
+  return {b->ctx->model_gen, b->precision, lg};
+}
+
+const Tune& tune_of(const fnet_batch* b) {
+  static const Tune defaults{};
+  if (b->launch != FNET_BATCHED) return defaults;
+  const auto it = b->ctx->tunes.find(tune_key(b));
+  return it == b->ctx->tunes.end() ? defaults : it->second;
+}
+
+}  // namespace fnet
+
 namespace {
 
 int fail(fnet_ctx* ctx, int code, const std::string& msg) {
@@ -154,7 +173,9 @@ int ensure_tiles(fnet_batch* b) {
   fnet_ctx* ctx = b->ctx;
   const DevModel& m = ctx->model;
   const int TA = mlp_tile_atoms(m.shape, net64(b->precision));
-  const int MW = mlp_mfma_waves(m.shape);  // 16 atoms per wave on the MFMA kernel
+  // 16 atoms per wave on the MFMA kernel; the tuned wave count when it fits
+  int MW = mlp_mfma_waves(m.shape);
+  if (MW && tune_of(b).mlp_waves > 0) MW = std::min(MW, tune_of(b).mlp_waves);
   if (b->tile_atoms == TA && b->mtile_waves == MW && b->tile_launch == b->launch && b->tile_sp)
     return FNET_OK;
   std::vector<int> perm, tsp, tstart, tcnt, msp, mstart, mcnt;
@@ -639,8 +660,10 @@ void drop_graph(fnet_batch* b) {
 // What the captured kernels' launch parameters depend on.
 std::vector<long long> graph_key(const fnet_batch* b) {
   const fnet_ctx* ctx = b->ctx;
+  const Tune& t = tune_of(b);
   return {b->gen, ctx->model_gen, b->precision, b->launch, b->nb_cap, b->max_nb, b->ncells,
-          b->N, b->S, b->ntiles, b->nmtiles};
+          b->N, b->S, b->ntiles, b->nmtiles, t.fwd_wpb, t.adj_wpb, t.mlp_waves, t.rbal,
+          t.persist};
 }
 
 int run_fast_step(fnet_batch* b) {
@@ -757,6 +780,9 @@ int fnet_ctx_create(int device, fnet_ctx** out) {
   if (scratch_init(device) != hipSuccess) return FNET_ERR_HIP;
   fnet_ctx* ctx = new fnet_ctx();
   ctx->device = device;
+  if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+      hipSuccess)
+    ctx->n_cu = 0;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return FNET_ERR_HIP;
@@ -1456,6 +1482,111 @@ int fnet_stage_times(fnet_ctx* ctx, double* ms, int n) {
     ms[k] = t;
   }
   return FNET_OK;
+}
+
+namespace {
+bool tune_valid(const int* k) {
+  return (k[0] == 0 || k[0] == 1 || k[0] == 2 || k[0] == 4) && (k[1] >= 0 && k[1] <= 2) &&
+         (k[2] == 0 || k[2] == 1 || k[2] == 2 || k[2] == 4) && (k[3] >= -1 && k[3] <= 1) &&
+         (k[4] == 0 || k[4] == 1);
+}
+}  // namespace
+
+int fnet_set_tune(fnet_ctx* ctx, fnet_batch* b, const int* knobs) {
+  if (!ctx || !b || !knobs) return FNET_ERR_ARG;
+  if (b->ctx != ctx) return fail(ctx, FNET_ERR_ARG, "batch belongs to another context");
+  if (b->launch != FNET_BATCHED) return fail(ctx, FNET_ERR_ARG, "tuning applies to batched launches");
+  if (!tune_valid(knobs)) return fail(ctx, FNET_ERR_ARG, "invalid launch parameters");
+  Tune t;
+  t.fwd_wpb = knobs[0];
+  t.adj_wpb = knobs[1];
+  t.mlp_waves = knobs[2];
+  t.rbal = knobs[3];
+  t.persist = knobs[4];
+  ctx->tunes[tune_key(b)] = t;
+  return FNET_OK;
+}
+
+int fnet_get_tune(fnet_ctx* ctx, const fnet_batch* b, int* knobs) {
+  if (!ctx || !b || !knobs) return FNET_ERR_ARG;
+  const Tune& t = tune_of(b);
+  knobs[0] = t.fwd_wpb;
+  knobs[1] = t.adj_wpb;
+  knobs[2] = t.mlp_waves;
+  knobs[3] = t.rbal;
+  knobs[4] = t.persist;
+  return FNET_OK;
+}
+
+int fnet_autotune(fnet_ctx* ctx, fnet_batch* b, int reps, double* ms, int* knobs) {
+  int rc = check_ready(ctx, b);
+  if (rc) return rc;
+  if (b->launch != FNET_BATCHED) return fail(ctx, FNET_ERR_ARG, "tuning applies to batched launches");
+  reps = std::max(1, std::min(reps, 50));
+  const int saved_graph = ctx->use_graph;
+  ctx->use_graph = 0;  // eager steps, with the stage events
+  const std::vector<long long> key = tune_key(b);
+  ctx->tunes.erase(key);
+  // time between events k0 and k1 of the steady-state step: the least of reps steps
+  auto measure = [&](int k0, int k1, double* out) -> int {
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      int e = fnet_run(ctx, b);
+      if (e) return e;
+      HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+      const bool eager = !b->last_graph && ctx->n_ev == 6;
+      e = validate_step(b);
+      if (e) return e;
+      if (!eager) continue;
+      float t = 0.f;
+      HIPCHECK(ctx, hipEventElapsedTime(&t, ctx->ev[k0], ctx->ev[k1]));
+      best = std::min(best, (double)t);
+    }
+    *out = best;
+    return FNET_OK;
+  };
+  double t_default = 0.0, t_tuned = 0.0, dummy = 0.0;
+  // two warm-up steps (the first sizes the list capacities), then the defaults
+  rc = measure(0, 5, &dummy);
+  if (!rc) rc = measure(0, 5, &dummy);
+  if (!rc) rc = measure(0, 5, &t_default);
+  Tune t;
+  // coordinate descent over the knobs, each on the stages it changes
+  auto choose = [&](int Tune::*field, std::initializer_list<int> cands, int k0, int k1) -> int {
+    int best_v = t.*field;
+    double best_t = 1e300;
+    for (int v : cands) {
+      t.*field = v;
+      ctx->tunes[key] = t;
+      double x = 0.0;
+      const int e = measure(k0, k1, &x);
+      if (e) return e;
+      if (x < best_t) {
+        best_t = x;
+        best_v = v;
+      }
+    }
+    t.*field = best_v;
+    ctx->tunes[key] = t;
+    return FNET_OK;
+  };
+  if (!rc) rc = choose(&Tune::fwd_wpb, {2, 1, 4}, 1, 2);
+  if (!rc) rc = choose(&Tune::rbal, {ctx->model.radial_resolved, 1 - ctx->model.radial_resolved}, 1, 2);
+  if (!rc) rc = choose(&Tune::adj_wpb, {1, 2}, 3, 4);
+  if (!rc) rc = choose(&Tune::persist, {0, 1}, 1, 4);
+  if (!rc && ctx->model.wpack && !net64(b->precision) && mlp_mfma_waves(ctx->model.shape))
+    rc = choose(&Tune::mlp_waves, {mlp_mfma_waves(ctx->model.shape), 2, 1}, 2, 3);
+  if (!rc) rc = measure(0, 5, &t_tuned);
+  ctx->use_graph = saved_graph;
+  if (rc) {
+    ctx->tunes.erase(key);
+    return rc;
+  }
+  if (ms) {
+    ms[0] = t_default;
+    ms[1] = t_tuned;
+  }
+  return fnet_get_tune(ctx, b, knobs);
 }
 
 int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const double* g_targets,

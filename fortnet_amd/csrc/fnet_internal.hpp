@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <type_traits>
 #include <string>
+#include <map>
 #include <vector>
 
 namespace fnet {
@@ -132,6 +133,20 @@ struct DevModel {
   bool valid = false;
 };
 
+// Launch parameters of the batched kernels chosen by the autotuner (fnet_autotune) for a model,
+// precision and batch size; 0 / -1 leave the built-in choice.
+struct Tune {
+  int fwd_wpb = 0;    // waves per block of the forward ACSF kernel (1, 2, 4)
+  int adj_wpb = 0;    // waves per block of the adjoint ACSF kernel (1, 2)
+  int mlp_waves = 0;  // waves of 16 atoms per block of the MFMA network kernels (1, 2, 4)
+  int rbal = -1;      // forward radial lanes in proportion to the element ranges (0, 1)
+  int persist = 0;    // ACSF kernels on a persistent grid (1)
+  bool operator==(const Tune& o) const {
+    return fwd_wpb == o.fwd_wpb && adj_wpb == o.adj_wpb && mlp_waves == o.mlp_waves &&
+           rbal == o.rbal && persist == o.persist;
+  }
+};
+
 struct Timer {
   hipEvent_t ev[8];
   bool made = false;
@@ -166,6 +181,9 @@ struct fnet_ctx {
   long long params_gen = 0;   // bumped by every change of the parameters (uploads, optimizer steps)
   double reg_strength = 0.0;  // elastic-net regularisation of the weight gradients
   double reg_alpha = 0.0;
+  int n_cu = 0;               // compute units of the device (persistent grids)
+  // autotuned launch parameters per (model generation, precision, log2 of the atom count)
+  std::map<std::vector<long long>, fnet::Tune> tunes;
 };
 
 struct fnet_batch {
@@ -285,6 +303,11 @@ struct fnet_batch {
 };
 
 namespace fnet {
+
+// The launch parameters for a batch: the tuned entry of its model, precision and size, else
+// the defaults.
+const Tune& tune_of(const fnet_batch* b);
+std::vector<long long> tune_key(const fnet_batch* b);
 
 // Distance stored with a neighbour entry (V.w).  FP64 rows: the square root of the FP64 squared
 // distance of the membership test.  FP32 rows (the FP32 path, held to its accuracy gates): the

@@ -248,6 +248,23 @@ int fnet_query(fnet_ctx* ctx);
  * n = number of entries written (<= 8).  A step replayed as a graph has only its total, in [0]. */
 int fnet_stage_times(fnet_ctx* ctx, double* ms, int n);
 
+/* Launch parameters of the batched kernels (no reference counterpart): knobs[5] = waves per
+ * block of the forward ACSF kernel (1, 2, 4; 0: default 2), of the adjoint (1, 2; 0: default 1),
+ * waves of 16 atoms per block of the MFMA network kernels (1, 2, 4; 0: the most that fit),
+ * radial lanes of the forward in proportion to the element ranges (0, 1; -1: when some radial
+ * function is species-resolved), ACSF kernels on a persistent grid (0, 1).  They are kept in the
+ * context per (model, precision, log2 of the atom count), so batches of about the same size
+ * share them; a new model starts from the defaults.  Results do not depend on them, except the
+ * radial lane split, which changes the order of the FP64 partial sums of the radial features. */
+int fnet_set_tune(fnet_ctx* ctx, fnet_batch* batch, const int* knobs);
+int fnet_get_tune(fnet_ctx* ctx, const fnet_batch* batch, int* knobs);
+/* Autotuner: times the batch's steady-state step (eager, reps steps per candidate, the least
+ * time of the stages a knob changes) over the knobs of fnet_set_tune one after another, keeps
+ * the fastest and stores it for the batch's model, precision and size.  ms[2] (may be NULL)
+ * receives the step time with the defaults and with the chosen parameters, knobs[5] the
+ * parameters. */
+int fnet_autotune(fnet_ctx* ctx, fnet_batch* batch, int reps, double* ms, int* knobs);
+
 #ifdef __cplusplus
 }
 #endif

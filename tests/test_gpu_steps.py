@@ -115,3 +115,61 @@ def test_bf16_modes(ctx, name, scale):
     e, f = errs[fa.FNET_BF16_FAST]
     assert np.isfinite(e) and np.isfinite(f) and e < 1e-2 and f < 1e-1, (e, f)
     b.close()
+
+
+TUNES = [dict(fwd_wpb=1), dict(fwd_wpb=4), dict(adj_wpb=2), dict(mlp_waves=1),
+         dict(mlp_waves=2), dict(persist=1), dict(fwd_wpb=4, adj_wpb=2, mlp_waves=2, persist=1)]
+
+
+@pytest.mark.parametrize("precision", [fa.FNET_FP64, fa.FNET_FP32, fa.FNET_BF16])
+@pytest.mark.parametrize("name,scale", [("water", 0.01), ("si_cells", 0.005)])
+def test_launch_parameters_do_not_change_results(ctx, name, scale, precision):
+    """Waves per block, MFMA tile waves and the persistent ACSF grids change how work is
+    scheduled, not what each centre or atom computes: energies and forces are bit-identical to
+    the default launch, through the first step and the steady state."""
+    print("workload", name, scale, "precision", precision, flush=True)
+    geos, ns = _zscored(ctx, name, scale)
+    ctx.set_graph(False)
+    b = ctx.batch(geos, precision)
+    b.run()
+    ref = _results(b)
+    for t in TUNES:
+        print("launch parameters", t, flush=True)
+        b.set_tune(**t)
+        assert {k: v for k, v in b.get_tune().items() if k in t} == t
+        b.run()
+        _same(ref, _results(b))
+        b.run()
+        _same(ref, _results(b))
+    b.set_tune()
+    b.close()
+
+
+@pytest.mark.parametrize("name,scale", [("water", 0.01), ("si_cells", 0.005)])
+def test_autotune_chooses_valid_parameters(ctx, name, scale):
+    """The autotuner returns parameters from its candidate sets, the default and tuned step
+    times, and leaves results within the FP32 gates of the FP64 path (its radial lane split may
+    reorder the FP64 partial sums of the radial features)."""
+    print("workload", name, scale, flush=True)
+    geos, ns = _zscored(ctx, name, scale)
+    b64 = ctx.batch(geos, fa.FNET_FP64)
+    E64, _, F64 = _results(b64)
+    b64.close()
+    b = ctx.batch(geos, fa.FNET_BF16)
+    params, t_def, t_tuned = b.autotune(reps=2)
+    print("autotune", params, "default %.3f ms tuned %.3f ms" % (t_def, t_tuned))
+    assert params["fwd_wpb"] in (1, 2, 4) and params["adj_wpb"] in (1, 2)
+    assert params["mlp_waves"] in (1, 2, 4) and params["rbal"] in (0, 1)
+    assert params["persist"] in (0, 1)
+    assert t_def > 0 and t_tuned > 0
+    assert b.get_tune() == params
+    # a second batch of the same model, precision and size class reuses the choice
+    b2 = ctx.batch(geos, fa.FNET_BF16)
+    assert b2.get_tune() == params
+    b2.close()
+    b.run()
+    Es, _, F = _results(b)
+    nat = np.array([g.n for g in geos], float)
+    assert (np.abs(Es[:, 0] - E64[:, 0]) / nat).max() <= E_TOL
+    assert np.sqrt(np.mean((F - F64) ** 2)) / fa.BOHR__AA <= F_TOL
+    b.close()
