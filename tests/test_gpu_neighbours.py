@@ -100,6 +100,20 @@ def test_cell_list_mid_size_cells(ctx):
     _compare(ctx, geos, 6.0 * fa.AA__BOHR)
 
 
+def test_mid_size_and_slab_cells(ctx):
+    """Cells 12-18 A wide and slab / needle cells of the same volumes: fewer than 3 bins of
+    width rc fit on some axis, so they take the direct scan over the reference's image ranges
+    (a cell-list search for them, with the stencil meeting one bin under several wraps, measured
+    1 ms slower on the 8192 Si cells); the rows must be the reference's entries in its order."""
+    rng = np.random.default_rng(7)
+    geos = [workloads.random_cell(rng, n, 2.33, 2.0, 14) for n in (90, 130, 200, 280)]
+    for shape, n in (((2.1, 1.0, 1.0), 160), ((1.0, 2.6, 1.3), 200), ((3.2, 1.05, 1.1), 260)):
+        g = workloads.random_cell(rng, n, 2.33, 2.0, 14)
+        f = np.array(shape) / np.prod(shape) ** (1.0 / 3.0)  # same volume, other aspect
+        geos.append(fa.Geometry(g.coords * f[None, :], g.atnum, g.species, g.latvecs * f[:, None]))
+    _compare(ctx, geos, 6.0 * fa.AA__BOHR)
+
+
 @pytest.mark.parametrize("precision", [fa.FNET_FP64, fa.FNET_FP32], ids=["fp64", "fp32"])
 def test_one_pass_search_and_overflow(ctx, precision):
     """After the first list of a batch, batched lists come from one search pass into rows sized by
