@@ -630,7 +630,13 @@ int enqueue_fast_step(fnet_batch* b, bool events) {
   HIPCHECK(ctx, launch_neigh_pad(b, rc, fp64, s));
   HIPCHECK(ctx, launch_cells_pad(b, rc, fp64, s));
   HIPCHECK(ctx, launch_row_max(b->cnt, b->N, b->d_maxnb, s));
-  HIPCHECK(ctx, launch_mirror(b, 0, b->N, layout_entries(b), s));
+  // The mirror entries need only the list: their search (latency-bound) runs on the side stream
+  // beside the ACSF forward and the networks (VALU- and matrix-bound), and the adjoint waits for
+  // it.  Under graph capture the event pair becomes a fork and a join of the graph.
+  HIPCHECK(ctx, hipEventRecord(ctx->fork_ev, s));
+  HIPCHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+  HIPCHECK(ctx, launch_mirror(b, 0, b->N, layout_entries(b), ctx->side));
+  HIPCHECK(ctx, hipEventRecord(ctx->join_ev, ctx->side));
   if (events) ev_record(ctx, 1);
   HIPCHECK(ctx, launch_acsf_fwd(b, m, 0, b->N, fp64, s));
   if (events) ev_record(ctx, 2);
@@ -638,6 +644,7 @@ int enqueue_fast_step(fnet_batch* b, bool events) {
   if (mfma) HIPCHECK(ctx, launch_mlp_mfma(b, m, 0, b->nmtiles, 1, s));
   else HIPCHECK(ctx, launch_mlp_fwd(b, m, 0, b->ntiles, net64(b->precision), 1, s));
   if (events) ev_record(ctx, 3);
+  HIPCHECK(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
   HIPCHECK(ctx, launch_acsf_adj(b, m, 0, b->N, fp64, s));
   HIPCHECK(ctx, launch_zero_int(b->d_norph, 1, s));
   HIPCHECK(ctx, launch_force_gather(b, m.nout, 0, b->N, s));
@@ -788,6 +795,12 @@ int fnet_ctx_create(int device, fnet_ctx** out) {
     return FNET_ERR_HIP;
   }
   for (int k = 0; k < 8; ++k) (void)hipEventCreate(&ctx->ev[k]);
+  if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+    fnet_ctx_destroy(ctx);
+    return FNET_ERR_HIP;
+  }
   *out = ctx;
   return FNET_OK;
 }
@@ -802,6 +815,12 @@ void fnet_ctx_destroy(fnet_ctx* ctx) {
   dfree(ctx->opt_states);
   dfree(ctx->opt_conv);
   for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ctx->ev[k]);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1577,6 +1596,12 @@ int fnet_autotune(fnet_ctx* ctx, fnet_batch* b, int reps, double* ms, int* knobs
   if (!rc && ctx->model.wpack && !net64(b->precision) && mlp_mfma_waves(ctx->model.shape))
     rc = choose(&Tune::mlp_waves, {mlp_mfma_waves(ctx->model.shape), 2, 1}, 2, 3);
   if (!rc) rc = measure(0, 5, &t_tuned);
+  // per-stage choices that do not add up to a faster step (noise, interactions) fall back to
+  // the defaults
+  if (!rc && t_tuned >= t_default) {
+    ctx->tunes.erase(key);
+    rc = measure(0, 5, &t_tuned);
+  }
   ctx->use_graph = saved_graph;
   if (rc) {
     ctx->tunes.erase(key);

@@ -159,6 +159,10 @@ struct fnet_batch;
 struct fnet_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // side stream of the steady-state step (the mirror entries run beside the ACSF forward and
+  // the networks) and the events that fork it from and join it to the main stream
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   fnet::DevModel model;
   hipEvent_t ev[8] = {};

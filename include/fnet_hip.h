@@ -260,7 +260,8 @@ int fnet_set_tune(fnet_ctx* ctx, fnet_batch* batch, const int* knobs);
 int fnet_get_tune(fnet_ctx* ctx, const fnet_batch* batch, int* knobs);
 /* Autotuner: times the batch's steady-state step (eager, reps steps per candidate, the least
  * time of the stages a knob changes) over the knobs of fnet_set_tune one after another, keeps
- * the fastest and stores it for the batch's model, precision and size.  ms[2] (may be NULL)
+ * the fastest and stores it for the batch's model, precision and size (the defaults when the
+ * chosen parameters do not make the whole step faster).  ms[2] (may be NULL)
  * receives the step time with the defaults and with the chosen parameters, knobs[5] the
  * parameters. */
 int fnet_autotune(fnet_ctx* ctx, fnet_batch* batch, int reps, double* ms, int* knobs);
