@@ -1635,10 +1635,11 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
   rc = do_reduce(b);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  // stage inputs: gt [S*ng], at [N*na], aw [N], dpw [S]; then the loss, a pad and the
-  // per-structure loss terms [2 S]
+  // stage inputs: gt [S*ng], at [N*na], aw [N], dpw [S]; then the loss, a pad, the
+  // per-structure loss terms [2 S] and the per-chunk partial sums [3 nchunks]
   const long long na = m.nout - n_global;
-  const long long need = (long long)b->S * n_global + b->N * na + b->N + 3LL * b->S + 4;
+  const long long need =
+      (long long)b->S * n_global + b->N * na + b->N + 3LL * b->S + 3LL * b->nchunks + 4;
   if (need > b->targets_cap) {
     dfree(b->d_targets);
     b->d_targets = nullptr;
@@ -1692,7 +1693,8 @@ int fnet_train_gradients(fnet_ctx* ctx, fnet_batch* b, int n_global, const doubl
     HIPCHECK(ctx, launch_regularize(m, ctx->d_grad, ctx->reg_strength / wsum, ctx->reg_alpha, s));
   double* dloss = dpw + b->S;
   HIPCHECK(ctx, launch_loss(b, m, n_global, gt, at, atomic_weights ? aw : nullptr,
-                            dp_weights ? dpw : nullptr, ctx->loss, dloss, dloss + 2, s));
+                            dp_weights ? dpw : nullptr, ctx->loss, dloss, dloss + 2,
+                            dloss + 2 + 2LL * b->S, s));
   if (grad)
     HIPCHECK(ctx, hipMemcpyAsync(grad, ctx->d_grad, sizeof(double) * np, hipMemcpyDeviceToHost, s));
   if (pred)

@@ -387,10 +387,11 @@ int grad_parts(const fnet_batch* b, const DevModel& m, int fp64);
 // MFMA weight gradient (FP32 / BF16 modes): blocks per MFMA tile, 0 without an MFMA path
 int mlp_grad_subtiles(const NetShape& sh, int tile_waves);
 hipError_t launch_mlp_grad_mfma(fnet_batch* b, const DevModel& m, hipStream_t s);
-// loss into out[0]; terms: [2 S] scratch (per-structure loss and weight)
+// loss into out[0]; scratch: terms [2 S] (per-structure loss and weight), part [3 nchunks]
+// (per energy chunk; the chunk tables of the energy reduction must be set up)
 hipError_t launch_loss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                        const double* at, const double* aw, const double* dpw, int loss,
-                       double* out, double* terms, hipStream_t s);
+                       double* out, double* terms, double* part, hipStream_t s);
 // elastic-net term (scale = strength / sum of datapoint weights) added to the device gradient
 hipError_t launch_regularize(const DevModel& m, double* grad, double scale, double alpha,
                              hipStream_t s);

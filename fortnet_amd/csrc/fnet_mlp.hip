@@ -399,21 +399,29 @@ __device__ __forceinline__ double item_loss(int loss, double sum_sq, double sum_
 
 // Loss of the batch (maLoss / mapLoss / msLoss / rmsLoss, loss.F90:370-700) in FP64: global
 // targets weighted by w_s * sum(atomic weights of s), atomic targets by w_s * atomic weight; the
-// result is divided by the summed weights.  k_loss_terms: one thread per structure writes its
-// weighted loss and weight; k_loss_sum adds them in a fixed order (thread t takes structures
-// t, t + 256, ... in order, then a fixed tree).
-__global__ void k_loss_terms(const double* __restrict__ Eatom, const double* __restrict__ Estruct,
-                             const long long* __restrict__ off, int S, int nout, int nglob,
-                             const double* __restrict__ gt, const double* __restrict__ at,
-                             const double* __restrict__ aw, const double* __restrict__ dpw,
-                             int loss, double* __restrict__ terms) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  double l = 0.0, nv = 0.0;
-  const int natomic = nout - nglob;
+// result is divided by the summed weights.  Three fixed-order levels: k_loss_chunks sums each
+// 2048-atom chunk of the energy reduction (one block: thread t takes atoms t, t + 256, ... in
+// order, then a fixed tree) into (atomic loss, atomic weight, summed atomic weights);
+// k_loss_terms adds a structure's chunks in order and its global term; k_loss_sum adds the
+// structures (thread t takes t, t + 256, ..., then a fixed tree).
+__global__ void __launch_bounds__(256) k_loss_chunks(const double* __restrict__ Eatom,
+                                                     const long long* __restrict__ off,
+                                                     const int* __restrict__ chunk_struct,
+                                                     const int* __restrict__ chunk_first,
+                                                     int nout, int nglob,
+                                                     const double* __restrict__ at,
+                                                     const double* __restrict__ aw,
+                                                     const double* __restrict__ dpw, int loss,
+                                                     double* __restrict__ part) {
+  __shared__ double r0[256], r1[256], r2[256];
+  const int c = blockIdx.x;
+  const int s = chunk_struct[c];
+  const long long a0 = off[s] + (long long)(c - chunk_first[s]) * kEnergyChunk;
+  const long long a1 = min(a0 + (long long)kEnergyChunk, off[s + 1]);
   const double w = dpw ? dpw[s] : 1.0;
-  double saw = 0.0;
-  for (long long i = off[s]; i < off[s + 1]; ++i) {
+  const int natomic = nout - nglob;
+  double l = 0.0, nv = 0.0, saw = 0.0;
+  for (long long i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
     const double awi = aw ? aw[i] : 1.0;
     saw += awi;
     if (natomic > 0) {
@@ -429,7 +437,40 @@ __global__ void k_loss_terms(const double* __restrict__ Eatom, const double* __r
       nv += w * awi;
     }
   }
+  r0[threadIdx.x] = l;
+  r1[threadIdx.x] = nv;
+  r2[threadIdx.x] = saw;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      r0[threadIdx.x] += r0[threadIdx.x + st];
+      r1[threadIdx.x] += r1[threadIdx.x + st];
+      r2[threadIdx.x] += r2[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * (long long)c] = r0[0];
+    part[3 * (long long)c + 1] = r1[0];
+    part[3 * (long long)c + 2] = r2[0];
+  }
+}
+
+__global__ void k_loss_terms(const double* __restrict__ Estruct,
+                             const int* __restrict__ chunk_first, int S, int nout, int nglob,
+                             const double* __restrict__ gt, const double* __restrict__ dpw,
+                             int loss, const double* __restrict__ part,
+                             double* __restrict__ terms) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double l = 0.0, nv = 0.0, saw = 0.0;
+  for (int c = chunk_first[s]; c < chunk_first[s + 1]; ++c) {
+    l += part[3 * (long long)c];
+    nv += part[3 * (long long)c + 1];
+    saw += part[3 * (long long)c + 2];
+  }
   if (nglob > 0) {
+    const double w = dpw ? dpw[s] : 1.0;
     double q = 0.0, a = 0.0, r = 0.0;
     for (int o = 0; o < nglob; ++o) {
       const double tv = gt[(long long)s * nglob + o];
@@ -605,10 +646,13 @@ hipError_t launch_lossgrad(fnet_batch* b, const DevModel& m, int n_global, const
 
 hipError_t launch_loss(fnet_batch* b, const DevModel& m, int n_global, const double* gt,
                        const double* at, const double* aw, const double* dpw, int loss,
-                       double* out, double* terms, hipStream_t s) {
+                       double* out, double* terms, double* part, hipStream_t s) {
+  if (b->nchunks > 0)
+    k_loss_chunks<<<b->nchunks, 256, 0, s>>>(b->Eatom, b->off, b->chunk_struct, b->chunk_first,
+                                             m.nout, n_global, at, aw, dpw, loss, part);
   if (b->S > 0)
-    k_loss_terms<<<(b->S + 255) / 256, 256, 0, s>>>(b->Eatom, b->Estruct, b->off, b->S, m.nout,
-                                                    n_global, gt, at, aw, dpw, loss, terms);
+    k_loss_terms<<<(b->S + 255) / 256, 256, 0, s>>>(b->Estruct, b->chunk_first, b->S, m.nout,
+                                                    n_global, gt, dpw, loss, part, terms);
   k_loss_sum<<<1, 256, 0, s>>>(terms, b->S, out);
   return hipGetLastError();
 }

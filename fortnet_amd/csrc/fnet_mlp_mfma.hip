@@ -81,6 +81,23 @@ __device__ __forceinline__ float actd(int id, float x) {
   }
 }
 
+// f(x) and f'(x) together: tanh and sigmoid share their transcendental (the same values as actf
+// and actd, bit for bit), the others are evaluated one after the other.
+__device__ __forceinline__ void actfd(int id, float x, float& f, float& d) {
+  if (id == 0) {
+    const float t = tanhf(x);
+    f = t;
+    d = 1.0f - t * t;
+  } else if (id == 1) {
+    const float s = 1.0f / (1.0f + expf(-x));
+    f = s;
+    d = s * (1.0f - s);
+  } else {
+    f = actf(id, x);
+    d = actd(id, x);
+  }
+}
+
 struct MfmaArgs {
   NetShape sh;
   const float* wpack;
@@ -254,8 +271,10 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
         const int ob = q >> 2, r = q & 3;
         float* slot = zs + (ob * 64 + lane) * 4 + r;
         const float z = *slot;
-        X[j * ldx + 16 * ob + 4 * g + r] = actf(act, z);
-        *slot = actd(act, z);
+        float f, d;
+        actfd(act, z, f, d);
+        X[j * ldx + 16 * ob + 4 * g + r] = f;
+        *slot = d;
       }
       wave_sync();
     }
