@@ -175,12 +175,12 @@ struct AcsfArgs {
 // (forward only) or the FP64 adjoint accumulator (adjoint only), the neighbour index and element
 // code, and the start and length of each element's range.  The slice is kept small: the number of
 // resident waves per CU, and with it the adjoint's speed, follows the LDS it takes (a slice 1.5x
-// larger made the adjoint 1.3x slower on the water boxes).  With paired groups the adjoint slice
-// also holds pb, the partner group's pa.
+// larger made the adjoint 1.3x slower on the water boxes).  With paired groups the slice also
+// holds pb, the partner group's pa.
 template <typename T, bool ADJ>
 __host__ __device__ size_t slice_bytes(int M, int pairs = 0) {
   size_t b = 2 * sizeof(typename Vec4<T>::type) * M;  // d, pa
-  if (ADJ && pairs) b += sizeof(typename Vec4<T>::type) * M;  // pb
+  if (pairs) b += sizeof(typename Vec4<T>::type) * M;  // pb
   if (!ADJ) b += 64 * sizeof(double) + sizeof(T) * M;  // radial partial sums, fc
   if (ADJ) b += 3 * sizeof(double) * M;               // gs
   b += 2 * sizeof(int) * M;                           // j, zc
@@ -230,7 +230,7 @@ __device__ Slice<T> make_slice(unsigned char* p, int M, int pairs) {
   s.pa = (V*)p;
   p += sizeof(V) * M;
   s.pb = nullptr;
-  if (ADJ && pairs) {
+  if (pairs) {
     s.pb = (V*)p;
     p += sizeof(V) * M;
   }
@@ -914,6 +914,104 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
   }
 }
 
+// Values of two G5 ladder groups over the same lists with the same cutoff and lambdas but another
+// eta, in one walk: the loads of the entries' geometry, the cosine and the ladder powers are
+// shared; each group has its own radial product (pa.y for the first group, pb.y for the partner)
+// and its own FP64 sums, which get exactly the terms, in the same order, that pairs_fwd_lad adds
+// for the group alone.
+template <typename T, int MX, int MASK, bool SAME>
+__device__ __forceinline__ void pairs_fwd_lad2(const Slice<T>& sl, int lane, int A0, int LA,
+                                               int B0, int LB, T lam0, T lam1,
+                                               double (&acc)[2][2][MX + 1]) {
+  using V = typename Vec4<T>::type;
+  PairWalk2<SAME> it;
+  it.init(lane, A0, LA, B0, LB);
+  int ej, ek1, ek2;
+  bool v1, v2;
+  const T w = SAME ? (T)2 : (T)1;
+  while (it.next(ej, ek1, ek2, v1, v2)) {
+    const V dj = sl.d[ej], pj = sl.pa[ej];
+    const T qj = sl.pb[ej].y;
+    const V dk1 = sl.d[ek1], pk1 = sl.pa[ek1];
+    const V dk2 = sl.d[ek2], pk2 = sl.pa[ek2];
+    const T qk1 = sl.pb[ek1].y, qk2 = sl.pb[ek2].y;
+    const T c1 = value_cos(dj, dk1, pj.x, pk1.x);
+    const T c2 = value_cos(dj, dk2, pj.x, pk2.x);
+    const T a1 = v1 ? w * pj.y * pk1.y : (T)0, a2 = v2 ? w * pj.y * pk2.y : (T)0;
+    const T b1 = v1 ? w * qj * qk1 : (T)0, b2 = v2 ? w * qj * qk2 : (T)0;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const T lam = l ? lam1 : lam0;
+      T p1 = (T)1 + lam * c1, p2 = (T)1 + lam * c2;
+      if (MASK & 1) {
+        acc[0][l][0] += (double)(a1 * p1 + a2 * p2);
+        acc[1][l][0] += (double)(b1 * p1 + b2 * p2);
+      }
+#pragma unroll
+      for (int m = 1; m <= MX; ++m) {
+        p1 = p1 * p1;
+        p2 = p2 * p2;
+        if (MASK & (1 << m)) {
+          acc[0][l][m] += (double)(a1 * p1 + a2 * p2);
+          acc[1][l][m] += (double)(b1 * p1 + b2 * p2);
+        }
+      }
+    }
+  }
+}
+
+// Values of a G5 ladder group (ga) and its partner (gb) of another eta; sl.pa holds ga's
+// per-entry factors, sl.pb gb's.
+template <typename T, int MX, int MASK>
+__device__ void group_lad_pair_fwd(CGroup& ga, CGroup& gb, const Slice<T>& sl, int lane,
+                                   long long i, int A0, int L1, int B0, int L2, double* G,
+                                   int nG) {
+  using V = typename Vec4<T>::type;
+  const T lam0 = (T)ga.lamv[0], lam1 = (T)ga.lamv[1];
+  double acc[2][2][MX + 1];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) acc[h][l][m] = 0.0;
+  if (ga.same) {
+    // j == k terms of both groups: cos(theta) = 1 (acsf.F90:1420-1432)
+    for (int e = lane; e < L1; e += 64) {
+      const T ea = sl.pa[A0 + e].y, eb = sl.pb[A0 + e].y;
+      const T ba = ea * ea, bb = eb * eb;
+      const T cjj = diag_cos(sl.d[A0 + e], (T)0);
+#pragma unroll
+      for (int l = 0; l < 2; ++l) {
+        T p = (T)1 + (l ? lam1 : lam0) * cjj;
+#pragma unroll
+        for (int m = 0; m <= MX; ++m) {
+          if (m > 0) p = p * p;
+          acc[0][l][m] += (double)(ba * p);
+          acc[1][l][m] += (double)(bb * p);
+        }
+      }
+    }
+    pairs_fwd_lad2<T, MX, MASK, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, acc);
+  } else {
+    pairs_fwd_lad2<T, MX, MASK, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, acc);
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    CGroup& gr = h ? gb : ga;
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m <= MX; ++m) {
+        const int q = gr.smem[5 * l + m];
+        if (q >= 0) {
+          const double s = wave_sum(acc[h][l][m]);
+          if (lane == 0) G[i * nG + gr.col[q]] = s * gr.scale[q];
+        }
+      }
+  }
+}
+
 // Adjoint of two G5 ladder groups over the same lists with the same cutoff, lambdas and exponent
 // slots but another eta, in one walk.  Per pair, the cosine, the ladder powers, the loads of the
 // shared entry data and the gradient vectors with their accumulator updates are shared; each
@@ -1030,17 +1128,16 @@ __device__ void group_lad_pair(CGroup& ga, CGroup& gb, const Slice<T>& sl,
   }
 }
 
-// One ladder group, or (adjoint, pg set) a G5 group with its partner.
+// One ladder group, or (pg set) a G5 group with its partner.
 template <typename T, bool ADJ, int MX, int MASK>
 __device__ __forceinline__ void lad_one(CGroup& grp, CGroup* pg,
                                         const Slice<T>& sl, int lane, long long i, int A0, int L1,
                                         int B0, int L2, T rc, T eta, T pi, T pirc,
                                         cdouble* cG, double* G, int nG) {
-  if constexpr (ADJ) {
-    if (pg) {
-      group_lad_pair<T, MX, MASK>(grp, *pg, sl, lane, A0, L1, B0, L2, cG);
-      return;
-    }
+  if (pg) {
+    if constexpr (ADJ) group_lad_pair<T, MX, MASK>(grp, *pg, sl, lane, A0, L1, B0, L2, cG);
+    else group_lad_pair_fwd<T, MX, MASK>(grp, *pg, sl, lane, i, A0, L1, B0, L2, G, nG);
+    return;
   }
   group_lad<T, ADJ, MX, MASK>(grp, sl, lane, i, A0, L1, B0, L2, rc, eta, pi, pirc, cG, G, nG);
 }
@@ -1083,7 +1180,7 @@ __device__ __forceinline__ double radial_sum(const AcsfArgs& a, const Slice<T>& 
 }
 
 // LAD: every angular group has the exponent-ladder form (AngGroup::lad > 0); MQ is then unused.
-// PAIRS (adjoint with ladder groups only): paired groups are evaluated together (own instance, so
+// PAIRS (ladder groups only): paired groups are evaluated together (own instance, so
 // that the kernels of models without pairs keep their register budget).  R64 (FP32 forward
 // only): radial terms in FP64 (radial_sum), also an instance of its own.
 template <typename T, bool ADJ, int MQ, bool LAD, bool PAIRS, bool R64 = false>
@@ -1300,6 +1397,9 @@ __device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i
     for (int g = 0; g < a.nAG; ++g) {
       CGroup& grp = ((CGroup*)a.groups)[g];
       const int nm = grp.nmem;
+      if constexpr (PAIRS && !ADJ) {
+        if (grp.follower) continue;  // evaluated together with its partner
+      }
       T A[MQ];
       if (ADJ) {
         // a group whose dE/dG are all zero adds nothing to the forces
@@ -1364,7 +1464,7 @@ __device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i
           if (PAIRS && pg) {
             const T ex2 = fexp(-eta2 * r * r);
             E2 = ex2 * fc;
-            Ep2 = ex2 * (dfc - (T)2 * eta2 * r * fc);
+            if (ADJ) Ep2 = ex2 * (dfc - (T)2 * eta2 * r * fc);
           }
         }
         V pa;
@@ -1551,11 +1651,19 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
   acsf_wave<float, ADJ, 1, true, false, false, true>(a);
 }
 
-// FP32 ladder adjoint with paired groups: 102 registers without spilling at 4 waves per SIMD
+// FP32 ladder adjoint with paired groups (68 registers: 7 waves per SIMD; the persistent
+// instance needs 98)
 template <bool PERSIST = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
     k_acsf_lad32_pairs(AcsfArgs a) {
   acsf_wave<float, true, 1, true, true, false, PERSIST>(a);
+}
+
+// FP32 ladder forward with paired groups: bounded to 80 registers (6 waves per SIMD; the
+// compiler's own choice, 87, leaves 5), which it meets without spilling
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))
+    k_acsf_lad32_fpairs(AcsfArgs a) {
+  acsf_wave<float, false, 1, true, true>(a);
 }
 
 // FP32 forward with the radial terms in FP64 (models with an ill-conditioned radial z-score)
@@ -1571,8 +1679,11 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   if (n <= 0) return hipSuccess;
   const Tune& tn = tune_of(b);
   const int M = slice_entries(b->max_nb);
-  // paired ladder groups share one adjoint walk when the larger slice still fits a block
-  const int pairs = (ADJ && LAD && m.has_pairs && slice_bytes<T, ADJ>(M, 1) <= 64 * 1024) ? 1 : 0;
+  // paired ladder groups share one walk when the larger slice still fits a block (the forward
+  // with an FP64 radial part and the persistent forward keep the unpaired kernels)
+  const int pairs = (LAD && m.has_pairs && slice_bytes<T, ADJ>(M, 1) <= 64 * 1024 &&
+                     (ADJ || !(std::is_same<T, float>::value && (m.radial64 || tn.persist))))
+                        ? 1 : 0;
   const size_t sb = slice_bytes<T, ADJ>(M, pairs);
   // waves per block: the tuned value, else 1 (adjoint) / 2 (forward) (water boxes: adjoint
   // 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 / 2.25 ms with 2 / 4)
@@ -1630,6 +1741,12 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
       if constexpr (std::is_same<T, float>::value)
         return ps ? go(k_acsf_lad32_pairs<true>, true) : go(k_acsf_lad32_pairs<false>, false);
       else return go(k_acsf<T, true, 1, true, true>, false);
+    }
+  }
+  if constexpr (!ADJ && LAD) {
+    if (pairs) {
+      if constexpr (std::is_same<T, float>::value) return go(k_acsf_lad32_fpairs, false);
+      else return go(k_acsf<T, false, 1, true, true>, false);
     }
   }
   if constexpr (!ADJ && std::is_same<T, float>::value) {
