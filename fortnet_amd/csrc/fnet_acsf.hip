@@ -165,7 +165,8 @@ struct AcsfArgs {
   double* F;        // [N][nout][3] (adjoint): the centre's own sum of its entries' gradients
   double* gvec;     // [entries][nout][3] (adjoint): gradient of E with respect to each entry's
                     // image position, stored at the index of the entry's mirror (the entry of
-                    // the neighbour's row that points back), or at its own index without one
+                    // the neighbour's row that points back), or at its own index without one;
+                    // FP64 values from the FP64 kernels, FP32 values from the FP32 ones
   const int* mir;   // [entries] mirror entries (-1: none)
 };
 
@@ -1589,10 +1590,20 @@ __device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i
                      gz = (double)sl.gs[3 * e + 2];
         const long long own = b + sl.j[e];
         const int me = a.mir[own];
-        double* gv = a.gvec + ((me >= 0 ? (long long)me : own) * a.nout + t) * 3;
-        gv[0] = gx;
-        gv[1] = gy;
-        gv[2] = gz;
+        const long long slot = ((me >= 0 ? (long long)me : own) * a.nout + t) * 3;
+        if constexpr (std::is_same<T, double>::value) {
+          double* gv = a.gvec + slot;
+          gv[0] = gx;
+          gv[1] = gy;
+          gv[2] = gz;
+        } else {
+          // FP32 kernels store the entry gradients in FP32 (half the bytes of the scattered
+          // stores and of the gather); the centre's own sum below keeps the FP64 accumulators
+          float* gv = (float*)a.gvec + slot;
+          gv[0] = (float)gx;
+          gv[1] = (float)gy;
+          gv[2] = (float)gz;
+        }
         sx += gx;
         sy += gy;
         sz += gz;

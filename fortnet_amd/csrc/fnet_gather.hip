@@ -9,7 +9,9 @@
 // so every atom gathers -g over its own row's slots, contiguously and in row order: the result
 // does not depend on scheduling, and the force array is read and written once per atom instead
 // of taking three FP64 atomics per entry (storing g_e at e's own slot and reading it through the
-// mirror index gave the same numbers but 5 GB of scattered reads per step on 44 M entries).
+// mirror index gave the same numbers but 5 GB of scattered reads per step on 44 M entries).  The
+// FP32 modes store g_e in FP32 (it is the sum of FP32 pair terms, kept in FP64 only while it is
+// accumulated); the gather sums in FP64 in both.
 //
 // The mirror of entry (i, j, n) is found in row j by binary search.  Rows are in the reference's
 // order (dynneighlist.F90:233-320): the central cell first, then the lattice points in
@@ -112,10 +114,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // F_i += -sum over row i's slots of g (each holding the gradient of the slot's mirror entry), per
-// output, in row order; entries of row i without a mirror (whose slot holds their own gradient)
-// go to the orphan list.
+// output, in row order, in FP64; entries of row i without a mirror (whose slot holds their own
+// gradient) go to the orphan list.  G: the type the adjoint stored g in (its own precision).
+template <typename G>
 __global__ void __launch_bounds__(256)
-    k_force_gather(RowLayout rl, const int* __restrict__ mir, const double* __restrict__ gvec,
+    k_force_gather(RowLayout rl, const int* __restrict__ mir, const G* __restrict__ gvec,
                    long long a0, long long a1, int nout, double* __restrict__ F,
                    int* __restrict__ orph, int* __restrict__ norph) {
   const int lane = threadIdx.x & 63;
@@ -129,10 +132,10 @@ __global__ void __launch_bounds__(256)
     double x = 0.0, y = 0.0, z = 0.0;
     for (int e = lane; e < len; e += 64) {
       if (mir[bi + e] >= 0) {
-        const double* g = gvec + ((bi + e) * nout + t) * 3;
-        x -= g[0];
-        y -= g[1];
-        z -= g[2];
+        const G* g = gvec + ((bi + e) * nout + t) * 3;
+        x -= (double)g[0];
+        y -= (double)g[1];
+        z -= (double)g[2];
       }
     }
     x = wave_sum(x);
@@ -148,9 +151,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // Entries without a mirror: -g to the neighbour's force.
+template <typename G>
 __global__ void __launch_bounds__(256)
     k_force_orphans(const int* __restrict__ norph, const int* __restrict__ orph,
-                    const int* __restrict__ nj, const double* __restrict__ gvec, int nout,
+                    const int* __restrict__ nj, const G* __restrict__ gvec, int nout,
                     double* __restrict__ F) {
   const int cnt = *norph;
   for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
@@ -158,7 +162,7 @@ __global__ void __launch_bounds__(256)
     const long long j = nj[e];
     for (int t = 0; t < nout; ++t)
       for (int c = 0; c < 3; ++c)
-        unsafeAtomicAdd(&F[(j * nout + t) * 3 + c], -gvec[(e * nout + t) * 3 + c]);
+        unsafeAtomicAdd(&F[(j * nout + t) * 3 + c], -(double)gvec[(e * nout + t) * 3 + c]);
   }
 }
 
@@ -207,9 +211,13 @@ hipError_t launch_force_gather(fnet_batch* b, int nout, long long a0, long long 
                                hipStream_t s) {
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
-  k_force_gather<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(layout_of(b), b->mir, b->gvec, a0, a1,
-                                                         nout, (double*)b->F, b->orph,
-                                                         b->d_norph);
+  const unsigned grid = (unsigned)((n + 3) / 4);
+  if (b->precision == FNET_FP64)
+    k_force_gather<double><<<grid, 256, 0, s>>>(layout_of(b), b->mir, b->gvec, a0, a1, nout,
+                                                (double*)b->F, b->orph, b->d_norph);
+  else
+    k_force_gather<float><<<grid, 256, 0, s>>>(layout_of(b), b->mir, (const float*)b->gvec, a0,
+                                               a1, nout, (double*)b->F, b->orph, b->d_norph);
   return hipGetLastError();
 }
 
@@ -239,7 +247,12 @@ void* scratch_ptr(int device) {
 
 hipError_t launch_force_orphans(fnet_batch* b, int nout, hipStream_t s) {
   const int* nj = b->nl_padded ? b->pnj : b->nj;
-  k_force_orphans<<<1, 256, 0, s>>>(b->d_norph, b->orph, nj, b->gvec, nout, (double*)b->F);
+  if (b->precision == FNET_FP64)
+    k_force_orphans<double><<<1, 256, 0, s>>>(b->d_norph, b->orph, nj, b->gvec, nout,
+                                               (double*)b->F);
+  else
+    k_force_orphans<float><<<1, 256, 0, s>>>(b->d_norph, b->orph, nj, (const float*)b->gvec,
+                                              nout, (double*)b->F);
   return hipGetLastError();
 }
 

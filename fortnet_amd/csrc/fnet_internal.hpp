@@ -270,7 +270,7 @@ struct fnet_batch {
   double* lossgrad = nullptr;  // [N][nout]
   // force assembly (fnet_gather.hip): per-entry adjoint gradients and mirror entries, sized for
   // the list layout (padded rows or CSR)
-  double* gvec = nullptr;      // [entries][nout][3]
+  double* gvec = nullptr;      // [entries][nout][3], FP64 values (FP64 mode) or FP32 values
   int* mir = nullptr;          // [entries] mirror entry (j, i, -n) of each entry, -1 if none
   int* orph = nullptr;         // [entries] entries without a mirror
   int* d_norph = nullptr;      // number of them
