@@ -491,7 +491,10 @@ int do_adjoint(fnet_batch* b) {
 
 // Cell-list set-up of the batch from the lattices: a periodic structure of at least
 // kCellMinAtoms atoms whose cell allows >= 3 bins of perpendicular width >= rc on every axis is
-// searched with cell lists, any other with the direct candidate scan.
+// searched with cell lists when that tests fewer candidates, priced at kCellCandidateCost, than
+// the direct scan (per centre: 27 bins of n / (nb_0 nb_1 nb_2) atoms, against n atoms in each of
+// the ~(1 + 2 w_a) images per axis that the scan's image pruning keeps, w_a = rc / width_a);
+// any other structure takes the direct scan.
 int setup_cells(fnet_batch* b, const double* lat, const int* per) {
   fnet_ctx* ctx = b->ctx;
   const double rc = ctx->model.rcmax;
@@ -508,6 +511,7 @@ int setup_cells(fnet_batch* b, const double* lat, const int* per) {
                        o(0, 2) * (o(1, 0) * o(2, 1) - o(2, 0) * o(1, 1));
     int nb[3];
     bool ok = true;
+    double direct = 1.0;  // images per atom that the direct scan tests
     for (int a = 0; a < 3; ++a) {
       // row a of inv(L) = (cofactor column a) / det
       const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
@@ -521,8 +525,11 @@ int setup_cells(fnet_batch* b, const double* lat, const int* per) {
       const double nbf = std::floor(1.0 / (w * (1.0 + 1e-9) + 1e-12));
       nb[a] = nbf > 1000.0 ? 1000 : (int)nbf;
       ok = ok && nb[a] >= 3;
+      direct *= 1.0 + 2.0 * w;
     }
     if (!ok) continue;
+    const double cells = kCellCandidateCost * 27.0 / ((double)nb[0] * nb[1] * nb[2]);
+    if (cells >= direct) continue;
     info[4 * s + 0] = nb[0];
     info[4 * s + 1] = nb[1];
     info[4 * s + 2] = nb[2];

@@ -335,7 +335,12 @@ hipError_t launch_row_max(const int* cnt, long long n, int* out, hipStream_t s);
 long long scan_tmp_size(long long n);
 
 // launchers (fnet_cells.hip)
-constexpr int kCellMinAtoms = 64; // structures at least this large use cell lists (if >= 3 bins fit)
+constexpr int kCellMinAtoms = 64; // structures at least this large may use cell lists
+// Cost of one cell-list candidate (bin walk, LDS sort into the reference's order) in units of
+// one direct-scan candidate: 8192 Si cells of 290-512 atoms (3 bins of width rc per axis, the
+// 27 stencil bins covering the whole cell, ~4 images per atom in the direct scan) took 1.83 ms
+// with cell lists and 1.69 ms with the direct scan.
+constexpr double kCellCandidateCost = 5.0;
 hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s);
 hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,

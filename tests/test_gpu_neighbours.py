@@ -94,9 +94,13 @@ def test_cell_list_path_exact(ctx):
 
 def test_cell_list_mid_size_cells(ctx):
     """Cells of a few hundred atoms whose width allows exactly 3 bins per axis: the 27 stencil
-    bins cover the whole cell and every candidate's image comes from the bin wrap."""
+    bins would cover the whole cell, which the direct scan does with fewer candidates, so these
+    take the direct scan; cells with 3 bins on some axes only (below and the 1152-atom diamond
+    above, 5 x 5 x 3 bins) take the cell lists, whose 3-bin axes get every image from the wrap."""
     rng = np.random.default_rng(3)
     geos = [workloads.random_cell(rng, n, 2.33, 2.0, 14) for n in (300, 420, 512)]
+    geos.append(W.si_diamond(7, 3, 4, 0.1, 12))  # 6 x 2 x 3 bins: direct scan (2 on one axis)
+    geos.append(W.si_diamond(8, 8, 4, 0.1, 13))  # 7 x 7 x 3 bins: cell lists
     _compare(ctx, geos, 6.0 * fa.AA__BOHR)
 
 
@@ -121,7 +125,7 @@ def test_one_pass_search_and_overflow(ctx, precision):
     that: the batch must fall back to the counted fill, and every list must be the reference's."""
     rng = np.random.default_rng(5)
     geos = [workloads.random_cell(rng, n, 2.33, 2.0, 14) for n in (13, 40, 300)]
-    geos.append(W.si_diamond(4, 4, 4, 0.1, 2))  # 512 atoms, cell-list search
+    geos.append(W.si_diamond(6, 6, 6, 0.1, 2))  # 1728 atoms: cell lists (5 and, compressed, 4 bins)
     rc = 6.0 * fa.AA__BOHR
     ctx.upload_model(_model(rc))
     b = ctx.batch(geos, precision)
