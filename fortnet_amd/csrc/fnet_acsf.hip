@@ -97,6 +97,42 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Wave sums of NA per-lane values at once (NA a power of two, at most 64), by halving exchange:
+// at the step of lane bit m (32, 16, ...) a lane keeps the half of its values that bit m of its
+// index selects and adds the partner's copy of that half, so after log2(NA) steps it holds one
+// value, index idx, summed over the lanes that differ in those bits; the remaining bits are
+// summed as in wave_sum.  NA sums take NA - 1 + log2(64 / NA) exchanges instead of 6 NA, and
+// every addition pairs the same partial sums as wave_sum's butterfly, so each total is
+// bit-identical to wave_sum of that value.  Lanes whose low log2(64 / NA) bits are zero hold the
+// NA totals once each.
+template <int NA, int H, int M>
+__device__ __forceinline__ void scatter_step(double (&v)[NA], int lane, int& idx) {
+  if constexpr (H >= 1) {
+    const bool up = (lane & M) != 0;
+#pragma unroll
+    for (int q = 0; q < H; ++q) {
+      const double keep = up ? v[q + H] : v[q];
+      const double send = up ? v[q] : v[q + H];
+      v[q] = keep + __shfl_xor(send, M, 64);
+    }
+    if (up) idx += H;
+    scatter_step<NA, H / 2, M / 2>(v, lane, idx);
+  }
+}
+
+template <int NA>
+__device__ __forceinline__ double wave_sum_scatter(double (&v)[NA], int lane, int& idx) {
+  static_assert(NA >= 1 && NA <= 64 && (NA & (NA - 1)) == 0, "NA: a power of two <= 64");
+  idx = 0;
+  scatter_step<NA, NA / 2, 32>(v, lane, idx);
+  double s = v[0];
+#pragma unroll
+  for (int m = 32 / NA; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  return s;
+}
+
+__host__ __device__ constexpr int pow2_at_least(int n) { return n <= 1 ? 1 : 2 * pow2_at_least((n + 1) / 2); }
+
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -867,16 +903,18 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
       if (g4) pairs_fwd_lad<T, MX, MASK, false, true>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
       else pairs_fwd_lad<T, MX, MASK, false, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, K, acc);
     }
+    // the 2 (MX + 1) slot sums in one exchange; slot (l, m) has flat index l (MX + 1) + m
+    constexpr int NA = pow2_at_least(2 * (MX + 1));
+    double v[NA];
 #pragma unroll
-    for (int l = 0; l < 2; ++l)
-#pragma unroll
-      for (int m = 0; m <= MX; ++m) {
-        const int q = grp.smem[5 * l + m];
-        if (q >= 0) {
-          const double s = wave_sum(acc[l][m]);
-          if (lane == 0) G[i * nG + grp.col[q]] = s * grp.scale[q];
-        }
-      }
+    for (int k = 0; k < NA; ++k) v[k] = k < 2 * (MX + 1) ? acc[k / (MX + 1)][k % (MX + 1)] : 0.0;
+    int k;
+    const double s = wave_sum_scatter<NA>(v, lane, k);
+    if ((lane & (64 / NA - 1)) == 0 && k < 2 * (MX + 1)) {
+      const int l = k / (MX + 1), m = k % (MX + 1);
+      const int q = grp.smem[5 * l + m];
+      if (q >= 0) G[i * nG + grp.col[q]] = s * grp.scale[q];
+    }
   } else {
     T c0[2][MX + 1], c1[2][MX + 1];
     T S0d = 0;
@@ -997,19 +1035,19 @@ __device__ void group_lad_pair_fwd(CGroup& ga, CGroup& gb, const Slice<T>& sl, i
   } else {
     pairs_fwd_lad2<T, MX, MASK, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, acc);
   }
+  // the 4 (MX + 1) slot sums of both groups in one exchange; flat index (2 h + l) (MX + 1) + m
+  constexpr int NS = 4 * (MX + 1), NA = pow2_at_least(NS);
+  double v[NA];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int k = 0; k < NA; ++k)
+    v[k] = k < NS ? acc[k / (2 * (MX + 1))][(k / (MX + 1)) % 2][k % (MX + 1)] : 0.0;
+  int k;
+  const double s = wave_sum_scatter<NA>(v, lane, k);
+  if ((lane & (64 / NA - 1)) == 0 && k < NS) {
+    const int h = k / (2 * (MX + 1)), l = (k / (MX + 1)) % 2, m = k % (MX + 1);
     CGroup& gr = h ? gb : ga;
-#pragma unroll
-    for (int l = 0; l < 2; ++l)
-#pragma unroll
-      for (int m = 0; m <= MX; ++m) {
-        const int q = gr.smem[5 * l + m];
-        if (q >= 0) {
-          const double s = wave_sum(acc[h][l][m]);
-          if (lane == 0) G[i * nG + gr.col[q]] = s * gr.scale[q];
-        }
-      }
+    const int q = gr.smem[5 * l + m];
+    if (q >= 0) G[i * nG + gr.col[q]] = s * gr.scale[q];
   }
 }
 

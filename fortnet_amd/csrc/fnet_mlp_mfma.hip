@@ -150,13 +150,32 @@ __device__ __forceinline__ void gemm_pair(const float* __restrict__ Af, int ob0,
                                           f4& c0, f4& c1) {
   const float* A0 = Af + (long long)ob0 * Kc * 256 + lane;
   const float* A1 = A0 + (long long)Kc * 256;
+  float n0[4], n1[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    n0[s] = A0[s * 64];
+    n1[s] = A1[s * 64];
+  }
 #pragma unroll 1
   for (int c = 0; c < Kc; ++c) {
     const f4 xb = *(const f4*)(X + j * ldx + 16 * c + 4 * g);
+    float u0[4], u1[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0[(c * 4 + s) * 64], xb[s], c0, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1[(c * 4 + s) * 64], xb[s], c1, 0, 0, 0);
+      u0[s] = n0[s];
+      u1[s] = n1[s];
+    }
+    if (c + 1 < Kc) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        n0[s] = A0[((c + 1) * 4 + s) * 64];
+        n1[s] = A1[((c + 1) * 4 + s) * 64];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(u0[s], xb[s], c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(u1[s], xb[s], c1, 0, 0, 0);
     }
   }
 }
@@ -181,17 +200,34 @@ __device__ __forceinline__ void gemm_pair_bf(const bf8* __restrict__ Ah, long lo
                                              int g, f4& c0, f4& c1) {
   const bf8* A0 = Ah + (long long)ob0 * Kc32 * 64 + lane;
   const bf8* A1 = A0 + (long long)Kc32 * 64;
+  bf8 na0 = A0[0], na1 = A1[0], nl0, nl1;
+  if (BF == 3) {
+    nl0 = A0[bsz];
+    nl1 = A1[bsz];
+  }
 #pragma unroll 1
   for (int c = 0; c < Kc32; ++c) {
     const float* xr = X + j * ldx + 32 * c + 8 * g;
     const f4 x0 = *(const f4*)xr, x1 = *(const f4*)(xr + 4);
     bf8 xh, xl;
     split8(x0, x1, xh, xl);
-    const bf8 a0 = A0[c * 64], a1 = A1[c * 64];
+    const bf8 a0 = na0, a1 = na1;
+    bf8 l0, l1;
+    if (BF == 3) {
+      l0 = nl0;
+      l1 = nl1;
+    }
+    if (c + 1 < Kc32) {
+      na0 = A0[(c + 1) * 64];
+      na1 = A1[(c + 1) * 64];
+      if (BF == 3) {
+        nl0 = A0[bsz + (c + 1) * 64];
+        nl1 = A1[bsz + (c + 1) * 64];
+      }
+    }
     c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, xh, c0, 0, 0, 0);
     c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, xh, c1, 0, 0, 0);
     if (BF == 3) {
-      const bf8 l0 = A0[bsz + c * 64], l1 = A1[bsz + c * 64];
       c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, xl, c0, 0, 0, 0);
       c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, xl, c1, 0, 0, 0);
       c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0, xh, c0, 0, 0, 0);
