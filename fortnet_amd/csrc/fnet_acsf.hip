@@ -1294,7 +1294,8 @@ __device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i
       }
     }
     if (e < n) {
-      const V d = nd[e];
+      V d = nd[e];
+      d.w = entry_r(d.w);  // the list holds the squared distance
       sl.d[pos] = d;
       sl.j[pos] = ADJ ? e : jj;  // adjoint: the entry's index in its row, for the output
       sl.zc[pos] = key < nZ ? key : -1;

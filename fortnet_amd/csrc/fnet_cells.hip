@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
         v.x = nd0;
         v.y = nd1;
         v.z = nd2;
-        entry_dist(v, dd);
+        entry_d2(v, dd);
         pnd[q] = v;
       }
     }

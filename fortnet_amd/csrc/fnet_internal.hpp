@@ -235,7 +235,7 @@ struct fnet_batch {
   long long scan_tmp_cap = 0;
   int* nj = nullptr;           // [P]
   int* nimg = nullptr;         // [P]
-  void* nd = nullptr;          // [P] double4 or float4: image position - centre, and r
+  void* nd = nullptr;          // [P] double4 or float4: image position - centre, and r^2
   long long pcap = 0;
   long long npairs = 0;
   int max_nb = 0;
@@ -313,12 +313,16 @@ namespace fnet {
 const Tune& tune_of(const fnet_batch* b);
 std::vector<long long> tune_key(const fnet_batch* b);
 
-// Distance stored with a neighbour entry (V.w).  FP64 rows: the square root of the FP64 squared
-// distance of the membership test.  FP32 rows (the FP32 path, held to its accuracy gates): the
-// FP32 square root of the rounded squared distance, which spares the search loops the FP64 square
-// root's refinement steps.  The membership test itself is FP64 in both.
-__device__ __forceinline__ void entry_dist(double4& v, double d2) { v.w = sqrt(d2); }
-__device__ __forceinline__ void entry_dist(float4& v, double d2) { v.w = sqrtf((float)d2); }
+// Distance of a neighbour entry.  The neighbour kernels store the squared distance of the FP64
+// membership test in V.w, in the list's precision (FP32 rows: rounded to FP32); the ACSF kernels
+// take the square root when they stage the entry (entry_r): FP64 rows the FP64 root, FP32 rows
+// (the FP32 path, held to its accuracy gates) the FP32 root of the rounded value.  Taking it
+// there costs one root per entry instead of one per tested candidate (the compiler evaluates
+// the root before the store's branch; ~27 candidates per entry on the Si cells).
+__device__ __forceinline__ void entry_d2(double4& v, double d2) { v.w = d2; }
+__device__ __forceinline__ void entry_d2(float4& v, double d2) { v.w = (float)d2; }
+__device__ __forceinline__ double entry_r(double w) { return sqrt(w); }
+__device__ __forceinline__ float entry_r(float w) { return sqrtf(w); }
 
 // launchers (fnet_neigh.hip)
 hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s);

@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(256)
       v.x = nd0;
       v.y = nd1;
       v.z = nd2;
-      entry_dist(v, d2);
+      entry_d2(v, d2);
       nd[pos] = v;
     }
     count += __popcll(mask);
