@@ -211,7 +211,7 @@ def serving_bench(rates, duration_s, max_requests, full):
     gen_s = time.perf_counter() - t0
     srv = serve.BatchingServer(ns, fa.FNET_FP32)
     per_atom = srv.calibrate(cells[:32])
-    srv.run([serve.Request(k, g, 0.0) for k, g in enumerate(cells[:64])])  # warm-up
+    srv.warm(cells[:64])  # both slots sized for 64 requests (~44 000 atoms) before timing
     out = {"workload": desc.replace("64 ternary", "fresh seeded ternary") +
            " (one new cell per request)", "precision": "fp32",
            "mean_atoms_per_request": float(np.mean([g.n for g in cells])),
@@ -294,7 +294,7 @@ def main():
                     help="20 000 requests per rate, a serial loop at every rate")
     ap.add_argument("--serve-rates", default="500,1000,2000,5000,10000,20000")
     ap.add_argument("--serve-duration", type=float, default=1.0)
-    ap.add_argument("--serve-max-requests", type=int, default=1500)
+    ap.add_argument("--serve-max-requests", type=int, default=4000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a plain 512 MiB device copy (x3) to scale HBM byte counters")
     args = ap.parse_args()

@@ -115,6 +115,16 @@ class BatchingServer:
         self.held[0] = atoms
         return self.per_atom
 
+    def warm(self, geos):
+        """Runs every slot once on ``geos`` (untimed), so that each slot's device buffers are
+        sized for a batch that large before requests are measured: a slot's first batch of a new
+        largest size allocates, which would otherwise land in the first latencies of a run."""
+        for k, slot in enumerate(self.slots):
+            slot.launch([Request(-1, g, 0.0) for g in geos])
+            slot.batch.sync()
+            slot.finish(time.perf_counter)
+            self.held[k] = max(self.held[k], sum(g.n for g in geos))
+
     def _budget(self, k):
         """Atoms slot k may take now: its resident buffers plus its share of the free memory."""
         free, _ = self.slots[k].ctx.device_memory()
