@@ -1647,15 +1647,10 @@ __device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i
         sy += gy;
         sz += gz;
       }
-      sx = wave_sum(sx);
-      sy = wave_sum(sy);
-      sz = wave_sum(sz);
-      if (lane == 0) {
-        double* f = a.F + (i * a.nout + t) * 3;
-        f[0] = sx;
-        f[1] = sy;
-        f[2] = sz;
-      }
+      double v[4] = {sx, sy, sz, 0.0};
+      int c;
+      const double sc = wave_sum_scatter<4>(v, lane, c);
+      if ((lane & 15) == 0 && c < 3) a.F[(i * a.nout + t) * 3 + c] = sc;
       wave_sync();
     }
   }

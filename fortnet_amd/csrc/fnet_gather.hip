@@ -138,15 +138,15 @@ __global__ void __launch_bounds__(256)
         z -= (double)g[2];
       }
     }
-    x = wave_sum(x);
-    y = wave_sum(y);
-    z = wave_sum(z);
-    if (lane == 0) {
-      double* f = F + (i * nout + t) * 3;
-      f[0] += x;
-      f[1] += y;
-      f[2] += z;
-    }
+    // x, y, z summed at once: lane bit 5 picks (x, y) or (z, 0), bit 4 one of the two
+    const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0;
+    double p0 = (u5 ? z : x) + __shfl_xor(u5 ? x : z, 32, 64);
+    double p1 = (u5 ? 0.0 : y) + __shfl_xor(u5 ? y : 0.0, 32, 64);
+    double q = (u4 ? p1 : p0) + __shfl_xor(u4 ? p0 : p1, 16, 64);
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) q += __shfl_xor(q, m, 64);
+    const int c = (u5 ? 2 : 0) + (u4 ? 1 : 0);
+    if ((lane & 15) == 0 && c < 3) F[(i * nout + t) * 3 + c] += q;
   }
 }
 
