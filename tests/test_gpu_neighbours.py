@@ -79,7 +79,7 @@ def test_small_cells_with_self_images(ctx, seed):
 
 
 def test_cell_list_path_exact(ctx):
-    """Structures of >= 1024 atoms with >= 3 bins per axis go through the cell-list kernels; the
+    """Structures of >= 1024 atoms with 5 x 5 x 3 bins of width rc take the cell-list kernels; the
     sorted rows must still be the reference's entries in the reference's order, including atoms
     moved out of the cell by lattice vectors (unwrapped coordinates)."""
     g = W.si_diamond(6, 6, 4, 0.1, 11)
