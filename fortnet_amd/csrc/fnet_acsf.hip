@@ -196,6 +196,7 @@ struct AcsfArgs {
   long long a0, a1;
   int maxnb;
   int persist;      // unused by the kernels (the persistent grid is an instance of its own)
+  int zone;         // the model has one element slot and every atom takes it
   double* G;        // [N][nG] (forward)
   const double* cG; // [N][nout][nG] (adjoint)
   double* F;        // [N][nout][3] (adjoint): the centre's own sum of its entries' gradients
@@ -1244,62 +1245,73 @@ __device__ __forceinline__ void acsf_centre(const AcsfArgs& a, const long long i
   const int nZ = a.nZ;  // element slots 0 .. nZ - 1; slot nZ: elements no function names
 
   // ---- stage the centre's entries in element order with the list-cutoff fc ----
-  int cnt[kMaxZ + 1];
+  // (one element slot that every atom takes: the entries keep their row order, and the element
+  // codes need not be read)
+  auto stage = [&](int pos, int e, int jj, int key) {
+    V d = nd[e];
+    d.w = entry_r(d.w);  // the list holds the squared distance
+    sl.d[pos] = d;
+    sl.j[pos] = ADJ ? e : jj;  // adjoint: the entry's index in its row, for the output
+    sl.zc[pos] = key < nZ ? key : -1;
+    if (!ADJ) sl.fc[pos] = (T)0.5 * (fcos(cutarg(d.w, pi, rcl, pircl)) + (T)1);
+  };
+  if (a.zone) {
+    if (lane < 2) {
+      sl.st[lane] = lane ? n : 0;
+      sl.nz[lane] = lane ? 0 : n;
+    }
+    for (int e = lane; e < n; e += 64) stage(e, e, ADJ ? 0 : a.nj[b + e], 0);
+  } else {
+    int cnt[kMaxZ + 1];
 #pragma unroll
-  for (int c = 0; c <= kMaxZ; ++c) cnt[c] = 0;
-  for (int e0 = 0; e0 < n; e0 += 64) {
-    const int e = e0 + lane;
-    int key = -1;
-    if (e < n) {
-      const int zc = a.zcode[a.nj[b + e]];
-      key = zc >= 0 ? zc : nZ;
+    for (int c = 0; c <= kMaxZ; ++c) cnt[c] = 0;
+    for (int e0 = 0; e0 < n; e0 += 64) {
+      const int e = e0 + lane;
+      int key = -1;
+      if (e < n) {
+        const int zc = a.zcode[a.nj[b + e]];
+        key = zc >= 0 ? zc : nZ;
+      }
+#pragma unroll
+      for (int c = 0; c <= kMaxZ; ++c)
+        if (c <= nZ) cnt[c] += __popcll(__ballot(key == c));
+    }
+    int start[kMaxZ + 1];
+    {
+      int s = 0;
+#pragma unroll
+      for (int c = 0; c <= kMaxZ; ++c) {
+        start[c] = s;
+        if (c <= nZ) s += cnt[c];
+      }
     }
 #pragma unroll
     for (int c = 0; c <= kMaxZ; ++c)
-      if (c <= nZ) cnt[c] += __popcll(__ballot(key == c));
-  }
-  int start[kMaxZ + 1];
-  {
-    int s = 0;
-#pragma unroll
-    for (int c = 0; c <= kMaxZ; ++c) {
-      start[c] = s;
-      if (c <= nZ) s += cnt[c];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c <= kMaxZ; ++c)
-    if (c == lane && c <= nZ) {
-      sl.st[c] = start[c];
-      sl.nz[c] = cnt[c];
-    }
-  int placed[kMaxZ + 1];
-#pragma unroll
-  for (int c = 0; c <= kMaxZ; ++c) placed[c] = 0;
-  for (int e0 = 0; e0 < n; e0 += 64) {
-    const int e = e0 + lane;
-    int key = -1, jj = 0;
-    if (e < n) {
-      jj = a.nj[b + e];
-      const int zc = a.zcode[jj];
-      key = zc >= 0 ? zc : nZ;
-    }
-    int pos = 0;
-#pragma unroll
-    for (int c = 0; c <= kMaxZ; ++c) {
-      if (c <= nZ) {
-        const unsigned long long m = __ballot(key == c);
-        if (key == c) pos = start[c] + placed[c] + __popcll(m & lanemask_lt(lane));
-        placed[c] += __popcll(m);
+      if (c == lane && c <= nZ) {
+        sl.st[c] = start[c];
+        sl.nz[c] = cnt[c];
       }
-    }
-    if (e < n) {
-      V d = nd[e];
-      d.w = entry_r(d.w);  // the list holds the squared distance
-      sl.d[pos] = d;
-      sl.j[pos] = ADJ ? e : jj;  // adjoint: the entry's index in its row, for the output
-      sl.zc[pos] = key < nZ ? key : -1;
-      if (!ADJ) sl.fc[pos] = (T)0.5 * (fcos(cutarg(d.w, pi, rcl, pircl)) + (T)1);
+    int placed[kMaxZ + 1];
+#pragma unroll
+    for (int c = 0; c <= kMaxZ; ++c) placed[c] = 0;
+    for (int e0 = 0; e0 < n; e0 += 64) {
+      const int e = e0 + lane;
+      int key = -1, jj = 0;
+      if (e < n) {
+        jj = a.nj[b + e];
+        const int zc = a.zcode[jj];
+        key = zc >= 0 ? zc : nZ;
+      }
+      int pos = 0;
+#pragma unroll
+      for (int c = 0; c <= kMaxZ; ++c) {
+        if (c <= nZ) {
+          const unsigned long long m = __ballot(key == c);
+          if (key == c) pos = start[c] + placed[c] + __popcll(m & lanemask_lt(lane));
+          placed[c] += __popcll(m);
+        }
+      }
+      if (e < n) stage(pos, e, jj, key);
     }
   }
   wave_sync();
@@ -1757,6 +1769,7 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   a.a1 = a1;
   a.maxnb = M;
   a.persist = tn.persist ? 1 : 0;
+  a.zone = b->zone;
   a.G = (double*)b->G;
   a.cG = (const double*)b->cG;
   a.F = (double*)b->F;

@@ -145,6 +145,29 @@ void invalidate(fnet_batch* b) {
   b->pending = false;
 }
 
+// Element slots of the batch's atoms under the current model (zcode_of of each atomic number),
+// uploaded on the context stream, and whether the model has one slot that every atom takes (the
+// ACSF staging then needs no element sort).
+int upload_zcode(fnet_batch* b) {
+  fnet_ctx* ctx = b->ctx;
+  const DevModel& m = ctx->model;
+  const size_t N = b->h_atnum.size();
+  std::vector<int> zc(N);
+  int one = m.nZ == 1;
+  for (size_t i = 0; i < N; ++i) {
+    zc[i] = zcode_of(m, b->h_atnum[i]);
+    one = one && zc[i] == 0;
+  }
+  b->zone = one;
+  b->zcode_gen = ctx->model_gen;
+  if (N)
+    HIPCHECK(ctx, hipMemcpyAsync(b->zcode, zc.data(), sizeof(int) * N, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  // the copy reads zc before this function returns
+  HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return FNET_OK;
+}
+
 // Packs the host copy of the parameters into MFMA fragments and uploads them (FP32 networks on
 // the matrix cores); a model whose shape the MFMA kernel does not take keeps wpack empty.
 int upload_wpack(fnet_ctx* ctx, DevModel& m, hipStream_t s) {
@@ -758,6 +781,10 @@ int check_ready(fnet_ctx* ctx, fnet_batch* b) {
   if (b->seen_model_gen != ctx->model_gen) {
     invalidate(b);
     b->seen_model_gen = ctx->model_gen;
+    if (b->zcode_gen != ctx->model_gen) {
+      const int rc = upload_zcode(b);
+      if (rc) return rc;
+    }
   } else if (b->seen_params_gen != ctx->params_gen) {
     b->have_E = b->have_F = false;
   }
@@ -1134,10 +1161,12 @@ int load_batch(fnet_ctx* ctx, fnet_batch* b, const fnet_batch_desc* d) {
     }
     for (long long i = d->offsets[s]; i < d->offsets[s + 1]; ++i) astruct[i] = s;
   }
+  int zone = m.nZ == 1;
   for (long long i = 0; i < N; ++i) {
     if (d->species[i] < 0 || d->species[i] >= m.nspecies)
       return fail(ctx, FNET_ERR_ARG, "species index out of range");
     zc[i] = zcode_of(m, d->atnum[i]);
+    zone = zone && zc[i] == 0;
   }
   hipStream_t s = ctx->stream;
   HIPCHECK(ctx, hipStreamSynchronize(s));
@@ -1227,6 +1256,9 @@ int load_batch(fnet_ctx* ctx, fnet_batch* b, const fnet_batch_desc* d) {
                                  s));
     HIPCHECK(ctx, hipMemcpyAsync(b->zcode, zc.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
   }
+  b->h_atnum.assign(d->atnum, d->atnum + N);
+  b->zone = zone;
+  b->zcode_gen = ctx->model_gen;
   if (N * d->n_ext)
     HIPCHECK(ctx, hipMemcpyAsync(b->ext, d->ext, sizeof(double) * N * d->n_ext,
                                  hipMemcpyHostToDevice, s));

@@ -209,7 +209,10 @@ struct fnet_batch {
   long long* off = nullptr;    // [S+1]
   int* atom_struct = nullptr;  // [N]
   int* species = nullptr;      // [N]
-  int* zcode = nullptr;        // [N]
+  int* zcode = nullptr;        // [N] element slot of each atom under the current model
+  std::vector<int> h_atnum;    // atomic numbers (host): element slots follow model uploads
+  int zone = 0;                // one element slot, taken by every atom (staging skips the sort)
+  unsigned long long zcode_gen = 0;  // model generation zcode was derived with
   double* ext = nullptr;       // [N][next]
   fnet::StructInfo* sinfo = nullptr; // [S]
   // per-species tiles of the network kernels
