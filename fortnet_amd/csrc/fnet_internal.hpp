@@ -78,6 +78,7 @@ struct NetShape {
 struct StructInfo {
   double inv[9];    // inverse lattice, inv[c*3+r] = inv(r,c)
   double smin[3], smax[3]; // extent of fractional coordinates
+  double bmin[3], bmax[3]; // Cartesian bounding box of the atoms
   double w[3];      // rc * |row_a(inv L)|: largest fractional offset of a neighbour
   int lo[3], hi[3]; // reference image ranges (latpointiter.F90:226-253)
   int periodic;

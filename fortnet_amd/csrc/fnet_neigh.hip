@@ -9,8 +9,11 @@
 //
 // Whole lattice images are skipped when no atom of the structure can reach the centre from
 // them: |row_a(inv L) . dr| <= rc |row_a(inv L)| bounds every fractional offset of a neighbour,
-// so an image n with n_a outside [s_a(i) - smax_a - w_a, s_a(i) - smin_a + w_a] holds none.
-// The exact test still decides every candidate that is looked at.
+// so an image n with n_a outside [s_a(i) - smax_a - w_a, s_a(i) - smin_a + w_a] holds none; and
+// an image whose translated Cartesian bounding box of the atoms lies farther than rc from the
+// centre holds none either (on cubic cells this drops the corner images of the first test's
+// box of images, ~10 % of the candidates on the Si cells).  The exact test still decides every
+// candidate that is looked at.
 #include <algorithm>
 
 #include "fnet_internal.hpp"
@@ -68,7 +71,7 @@ __global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ 
                                                      const int4* __restrict__ cellinfo,
                                                      StructInfo* __restrict__ sinfo) {
   __shared__ StructInfo si;
-  __shared__ double red[2][3][256];
+  __shared__ double red[2][6][256];
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -85,7 +88,12 @@ __global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ 
     }
   }
   __syncthreads();
-  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  // fractional extent (a = 0..2) and Cartesian bounding box (a = 3..5)
+  double mn[6], mx[6];
+  for (int a = 0; a < 6; ++a) {
+    mn[a] = 1e300;
+    mx[a] = -1e300;
+  }
   if (si.periodic && !cellinfo[s].w) {
     const long long a0 = off[s], a1 = off[s + 1];
     for (long long i = a0 + tid; i < a1; i += blockDim.x) {
@@ -95,16 +103,22 @@ __global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ 
         mn[a] = fmin(mn[a], f);
         mx[a] = fmax(mx[a], f);
       }
+      mn[3] = fmin(mn[3], x);
+      mx[3] = fmax(mx[3], x);
+      mn[4] = fmin(mn[4], y);
+      mx[4] = fmax(mx[4], y);
+      mn[5] = fmin(mn[5], z);
+      mx[5] = fmax(mx[5], z);
     }
   }
-  for (int a = 0; a < 3; ++a) {
+  for (int a = 0; a < 6; ++a) {
     red[0][a][tid] = mn[a];
     red[1][a][tid] = mx[a];
   }
   __syncthreads();
   for (int st = blockDim.x / 2; st > 0; st >>= 1) {
     if (tid < st) {
-      for (int a = 0; a < 3; ++a) {
+      for (int a = 0; a < 6; ++a) {
         red[0][a][tid] = fmin(red[0][a][tid], red[0][a][tid + st]);
         red[1][a][tid] = fmax(red[1][a][tid], red[1][a][tid + st]);
       }
@@ -115,6 +129,8 @@ __global__ void __launch_bounds__(256) k_struct_prep(const double* __restrict__ 
     for (int a = 0; a < 3; ++a) {
       si.smin[a] = red[0][a][0];
       si.smax[a] = red[1][a][0];
+      si.bmin[a] = red[0][a + 3][0];
+      si.bmax[a] = red[1][a + 3][0];
     }
     sinfo[s] = si;
   }
@@ -239,6 +255,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
           acc = acc + L[2 * 3 + r] * pv2;
           cv[r] = acc;
         }
+        // distance from the centre to the image's box of atoms; the margin covers the
+        // rounding of this estimate against the exact test's
+        const double ci[3] = {ci0, ci1, ci2};
+        double q2 = 0.0;
+        for (int r = 0; r < 3; ++r) {
+          const double g = fmax(0.0, fmax(si.bmin[r] + cv[r] - ci[r], ci[r] - si.bmax[r] - cv[r]));
+          q2 += g * g;
+        }
+        keep = keep && q2 <= rc2 * (1.0 + 1e-9) + 1e-12;
         tv = make_double4(cv[0], cv[1], cv[2], (double)(q > qorig ? q - 1 : q));
       }
       const unsigned long long km = __ballot(keep);
