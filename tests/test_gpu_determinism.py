@@ -177,3 +177,34 @@ def test_results_follow_parameter_and_model_uploads(ctx):
     assert np.array_equal(E3, fresh.energy()[0])
     fresh.close()
     b.close()
+
+
+def test_element_slots_follow_model_uploads(ctx):
+    """A batch keeps the atomic numbers of its atoms: after a model with another element table is
+    uploaded, its atoms' element slots are derived again, and its features, energies and forces
+    equal those of a fresh batch.  Si cells under a Si-only model (one element slot that every
+    atom takes: the staging without the element sort), then under a Si/O model whose radial
+    functions count Si and O neighbours apart (Si takes slot 1 of [O, Si], so slots kept from the
+    first model would count the Si neighbours as O), then under the first model again."""
+    geos, ns = _zscored(ctx, "si64", 0.03)
+    b = ctx.batch(geos, fa.FNET_FP64)
+    G0, E0, F0 = b.acsf(), b.energy()[0], b.forces()
+    rc = float(ns.g.rcut[0])
+    g = fa.GFunctions([2, 2, 2], [rc] * 3, eta=[0.01, 0.05, 0.01], z1=[14, 14, 8])
+    dims = [3, 4, 1]
+    ns2 = fa.Netstat(g, [14, 8], dims, 0, W.random_params(dims, 2, 7))
+    ctx.upload_model(ns2)
+    for prec in (fa.FNET_FP64, fa.FNET_FP32):
+        b.set_mode(prec)
+        G1, E1, F1 = b.acsf(), b.energy()[0], b.forces()
+        fresh = ctx.batch(geos, prec)
+        assert np.array_equal(G1, fresh.acsf())
+        assert np.array_equal(E1, fresh.energy()[0])
+        assert np.array_equal(F1, fresh.forces())
+        fresh.close()
+        assert (G1[:, 0] > 0).all() and (G1[:, 2] == 0).all()  # Si neighbours only
+    ctx.upload_model(ns)
+    b.set_mode(fa.FNET_FP64)
+    assert np.array_equal(G0, b.acsf())
+    assert np.array_equal(E0, b.energy()[0]) and np.array_equal(F0, b.forces())
+    b.close()
