@@ -138,7 +138,8 @@ __global__ void __launch_bounds__(256)
         z -= (double)g[2];
       }
     }
-    // x, y, z summed at once: lane bit 5 picks (x, y) or (z, 0), bit 4 one of the two
+    // x, y, z summed at once: lane bit 5 picks (x, y) or (z, 0), bit 4 one of the two; each
+    // addition pairs the same partial sums as a butterfly sum of that component would
     const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0;
     double p0 = (u5 ? z : x) + __shfl_xor(u5 ? x : z, 32, 64);
     double p1 = (u5 ? 0.0 : y) + __shfl_xor(u5 ? y : 0.0, 32, 64);
