@@ -21,6 +21,15 @@
 namespace fnet {
 namespace {
 
+// q / d for 0 <= q, d < 2^31 through the FP64 reciprocal rd = 1 / d: the estimate is within one of
+// the quotient, and one correction step makes it exact (no integer division sequence)
+__device__ __forceinline__ int udiv(int q, int d, double rd) {
+  int a = (int)((double)q * rd);
+  const long long r = (long long)q - (long long)a * d;
+  a += r >= d ? 1 : (r < 0 ? -1 : 0);
+  return a;
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
   return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 }
@@ -235,6 +244,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
     const int m1 = si.hi[0] - si.lo[0] + 1, m2 = si.hi[1] - si.lo[1] + 1,
               m3 = si.hi[2] - si.lo[2] + 1;
     const int nq = m1 * m2 * m3;
+    const int m23 = m2 * m3;
+    const double rm23 = 1.0 / (double)m23, rm3 = 1.0 / (double)m3;
     const int qorig = ((0 - si.lo[0]) * m2 + (0 - si.lo[1])) * m3 + (0 - si.lo[2]);
     const int qs = 64 / n, rs = 64 % n;
     for (int q0 = 0; q0 < nq; q0 += 64) {
@@ -242,8 +253,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
       bool keep = false;
       double4 tv = make_double4(0.0, 0.0, 0.0, 0.0);
       if (q < nq && q != qorig) {
-        const int p1 = si.lo[0] + q / (m2 * m3), p2 = si.lo[1] + (q / m3) % m2,
-                  p3 = si.lo[2] + q % m3;
+        // q = ((p1 - lo0) m2 + (p2 - lo1)) m3 + (p3 - lo2)
+        const int a1 = udiv(q, m23, rm23), rem = q - a1 * m23, a2 = udiv(rem, m3, rm3);
+        const int p1 = si.lo[0] + a1, p2 = si.lo[1] + a2, p3 = si.lo[2] + (rem - a2 * m3);
         keep = !(p1 < wlo[0] || p1 > whi[0] || p2 < wlo[1] || p2 > whi[1] || p3 < wlo[2] ||
                  p3 > whi[2]);
         const double pv0 = (double)p1, pv1 = (double)p2, pv2 = (double)p3;
