@@ -1036,19 +1036,22 @@ __device__ void group_lad_pair_fwd(CGroup& ga, CGroup& gb, const Slice<T>& sl, i
   } else {
     pairs_fwd_lad2<T, MX, MASK, false>(sl, lane, A0, L1, B0, L2, lam0, lam1, acc);
   }
-  // the 4 (MX + 1) slot sums of both groups in one exchange; flat index (2 h + l) (MX + 1) + m
-  constexpr int NS = 4 * (MX + 1), NA = pow2_at_least(NS);
-  double v[NA];
+  // the slot sums of each group in one exchange (group by group: fewer values live at once);
+  // slot (l, m) has flat index l (MX + 1) + m
+  constexpr int NS = 2 * (MX + 1), NA = pow2_at_least(NS);
 #pragma unroll
-  for (int k = 0; k < NA; ++k)
-    v[k] = k < NS ? acc[k / (2 * (MX + 1))][(k / (MX + 1)) % 2][k % (MX + 1)] : 0.0;
-  int k;
-  const double s = wave_sum_scatter<NA>(v, lane, k);
-  if ((lane & (64 / NA - 1)) == 0 && k < NS) {
-    const int h = k / (2 * (MX + 1)), l = (k / (MX + 1)) % 2, m = k % (MX + 1);
-    CGroup& gr = h ? gb : ga;
-    const int q = gr.smem[5 * l + m];
-    if (q >= 0) G[i * nG + gr.col[q]] = s * gr.scale[q];
+  for (int h = 0; h < 2; ++h) {
+    double v[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) v[k] = k < NS ? acc[h][k / (MX + 1)][k % (MX + 1)] : 0.0;
+    int k;
+    const double s = wave_sum_scatter<NA>(v, lane, k);
+    if ((lane & (64 / NA - 1)) == 0 && k < NS) {
+      const int l = k / (MX + 1), m = k % (MX + 1);
+      CGroup& gr = h ? gb : ga;
+      const int q = gr.smem[5 * l + m];
+      if (q >= 0) G[i * nG + gr.col[q]] = s * gr.scale[q];
+    }
   }
 }
 
@@ -1717,8 +1720,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 }
 
 // FP32 ladder forward with paired groups: bounded to 80 registers (6 waves per SIMD instead of 5),
-// which spills 14 of them, written and read back once per centre around the slot sums; on the
-// 8192 Si cells this still runs 1 % faster than 5 waves without spilling
+// which spills 6 of them, written and read back once per centre around the slot sums; on the
+// 8192 Si cells this runs faster than 5 waves without spilling
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))
     k_acsf_lad32_fpairs(AcsfArgs a) {
   acsf_wave<float, false, 1, true, true>(a);
