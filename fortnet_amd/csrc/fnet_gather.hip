@@ -308,17 +308,20 @@ extern "C" int fnet_forces_noalloc(fnet_ctx* ctx, fnet_batch* b) {
   unsigned char* base = (unsigned char*)scratch_ptr(ctx->device);
   if (!base) return noalloc_fail(ctx, FNET_ERR_STATE, "device scratch not set up");
   const long long entries = std::max<long long>(layout_entries(b), 1);
-  const long long gbytes = align256(entries * m.nout * 3 * (long long)sizeof(double));
+  const int fp64 = b->precision == FNET_FP64;
+  // per-entry gradients in the adjoint's precision (FP32 in the FP32 modes)
+  const long long gsize = fp64 ? (long long)sizeof(double) : (long long)sizeof(float);
+  const long long gbytes = align256(entries * m.nout * 3 * gsize);
   const long long ibytes = align256(entries * (long long)sizeof(int));
   if (gbytes + 2 * ibytes + 256 > kNoallocScratchBytes)
     return noalloc_fail(ctx, FNET_ERR_CAPACITY,
                         "neighbour list too large for the 32 MiB scratch of "
-                        "fnet_forces_noalloc (need (24 n_out + 8) bytes per list entry)");
+                        "fnet_forces_noalloc (need (24 n_out + 8) bytes per list entry in FP64, "
+                        "(12 n_out + 8) in the FP32 modes)");
   double* gvec = (double*)base;
   int* mir = (int*)(base + gbytes);
   int* orph = (int*)(base + gbytes + ibytes);
   int* norph = (int*)(base + gbytes + 2 * ibytes);
-  const int fp64 = b->precision == FNET_FP64;
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
   {

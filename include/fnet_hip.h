@@ -227,9 +227,9 @@ int fnet_optimizer_step(fnet_ctx* ctx, int kind, double* params_out, int* conver
  * reference: it is for callers that cannot allocate -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  It allocates nothing, does not synchronise
  * and does not copy: its per-entry scratch lives in a static device array of 32 MiB per GPU
- * ((24 n_out + 8) bytes per list entry, larger lists are refused with FNET_ERR_CAPACITY).  All
- * calls on one GPU share that array: calls that overlap on one GPU (two streams, two contexts)
- * race. */
+ * ((24 n_out + 8) bytes per list entry in FP64, (12 n_out + 8) in the FP32 modes; larger lists
+ * are refused with FNET_ERR_CAPACITY).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (two streams, two contexts) race. */
 int fnet_forces_noalloc(fnet_ctx* ctx, fnet_batch* batch);
 
 /* Free and total device memory in bytes (hipMemGetInfo): the serving loop admits requests by it.
