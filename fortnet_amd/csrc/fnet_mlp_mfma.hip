@@ -301,8 +301,10 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
     }
     wave_sync();  // every lane has read this layer's rows before they are overwritten
     if (hidden) {
-      // activation: z in the fragment slots -> next rows y = f(z), slots <- f'(z)
-#pragma unroll 1
+      // activation: z in the fragment slots -> next rows y = f(z), slots <- f'(z); four at a
+      // time, so that their LDS reads and transcendentals overlap (the kernel's occupancy is
+      // set by its LDS, not by these registers)
+#pragma unroll 4
       for (int q = 0; q < 4 * Nb; ++q) {
         const int ob = q >> 2, r = q & 3;
         float* slot = zs + (ob * 64 + lane) * 4 + r;
@@ -326,7 +328,7 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
     const float* col = a.params_f + (long long)sp * a.sh.nparams + a.sh.woff[m];
     if (m >= 1) {
       float* sl = FP + a.fpoff[m];
-#pragma unroll 1
+#pragma unroll 4
       for (int q = 0; q < 4 * Kb; ++q) {
         const int ib = q >> 2, r = q & 3, k = 16 * ib + 4 * g + r;
         float* slot = sl + (ib * 64 + lane) * 4 + r;
@@ -373,7 +375,7 @@ __global__ void __launch_bounds__(256) k_mlp_mfma(MfmaArgs a) {
     wave_sync();
     if (m - 1 >= 1) {
       // delta_{m-1} fragments -> rows for the next product
-#pragma unroll 1
+#pragma unroll 4
       for (int q = 0; q < 4 * Mb; ++q) {
         const int ib = q >> 2, r = q & 3;
         X[j * ldx + 16 * ib + 4 * g + r] = sl[(ib * 64 + lane) * 4 + r];
