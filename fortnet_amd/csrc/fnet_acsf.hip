@@ -1740,17 +1740,20 @@ hipError_t launch_mq(fnet_batch* b, const DevModel& m, long long a0, long long a
   if (n <= 0) return hipSuccess;
   const Tune& tn = tune_of(b);
   const int M = slice_entries(b->max_nb);
+  const size_t lmax = (size_t)b->ctx->lds_block;
   // paired ladder groups share one walk when the larger slice still fits a block (the forward
   // with an FP64 radial part and the persistent forward keep the unpaired kernels)
-  const int pairs = (LAD && m.has_pairs && slice_bytes<T, ADJ>(M, 1) <= 64 * 1024 &&
+  const int pairs = (LAD && m.has_pairs && slice_bytes<T, ADJ>(M, 1) <= lmax &&
                      (ADJ || !(std::is_same<T, float>::value && (m.radial64 || tn.persist))))
                         ? 1 : 0;
   const size_t sb = slice_bytes<T, ADJ>(M, pairs);
   // waves per block: the tuned value, else 1 (adjoint) / 2 (forward) (water boxes: adjoint
   // 4.75 / 4.83 / 5.9 ms with 1 / 2 / 4, forward 2.23 / 2.25 ms with 2 / 4)
   int wpb = ADJ ? (tn.adj_wpb > 0 ? tn.adj_wpb : 1) : (tn.fwd_wpb > 0 ? tn.fwd_wpb : 2);
+  // more than one wave per block while their slices fit 64 KiB; a longer row's slice takes a
+  // block of its own, up to the device's limit
   while (wpb > 1 && sb * wpb > 64 * 1024) wpb >>= 1;
-  if (sb * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
+  if (sb * wpb > lmax) return hipErrorInvalidConfiguration;
   AcsfArgs a;
   a.rowptr = b->rowptr;
   a.cnt = b->cnt;

@@ -352,7 +352,7 @@ int do_neighbours(fnet_batch* b) {
   b->max_nb = hv[1];
   if (b->max_nb > 65535)
     return fail(ctx, FNET_ERR_CAPACITY, "more than 65535 neighbours around one atom");
-  if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > 64 * 1024)
+  if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > (size_t)ctx->lds_block)
     return fail(ctx, FNET_ERR_CAPACITY,
                 "too many neighbours around one atom for the LDS staging (" +
                     std::to_string(b->max_nb) + ")");
@@ -624,7 +624,8 @@ bool fast_step_ready(fnet_batch* b) {
   const DevModel& m = ctx->model;
   if (b->launch != FNET_BATCHED || b->N <= 0 || b->nb_cap <= 0 || b->max_nb <= 0) return false;
   if (acsf_slice_entries(b->max_nb) > b->nb_cap) return false;
-  if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > 64 * 1024) return false;
+  if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > (size_t)ctx->lds_block)
+    return false;
   if (!ensure_padded(b)) return false;
   const bool was = b->nl_padded;
   b->nl_padded = true;
@@ -824,6 +825,13 @@ int fnet_ctx_create(int device, fnet_ctx** out) {
   if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
       hipSuccess)
     ctx->n_cu = 0;
+  // a block's LDS: 160 KiB on gfx950 (the whole CU's), which the per-centre staging of the ACSF
+  // kernels may take when a structure's rows are long (one wave per block then)
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+          hipSuccess &&
+      lds > 0)
+    ctx->lds_block = lds;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return FNET_ERR_HIP;

@@ -187,6 +187,7 @@ struct fnet_ctx {
   double reg_strength = 0.0;  // elastic-net regularisation of the weight gradients
   double reg_alpha = 0.0;
   int n_cu = 0;               // compute units of the device (persistent grids)
+  int lds_block = 64 * 1024;  // LDS bytes one block may allocate (the device's limit)
   // autotuned launch parameters per (model generation, precision, log2 of the atom count)
   std::map<std::vector<long long>, fnet::Tune> tunes;
 };

@@ -7,8 +7,10 @@
 * a two-atom periodic cell and a small cluster, next to them: ragged rows in one batch;
 * species-resolved lists where an element is absent around a centre: empty element ranges of
   radial functions and of both kinds of angular groups;
-* a structure too dense for the LDS staging of one centre: the host reports it as an error
-  before any ACSF kernel runs (every kernel that did run, the neighbour search, ends normally).
+* rows longer than 64 KiB of LDS staging take a block of their own (gfx950 gives one block up to
+  160 KiB): a three-atom cell of 1.7 A^3 whose centres have ~900 neighbour images each;
+* a structure too dense even for that: the host reports it as an error before any ACSF kernel
+  runs (every kernel that did run, the neighbour search, ends normally).
 
 FP64 is held to 1e-10 relative to the largest value (as the other oracle comparisons), FP32 to
 the accuracy gates (1e-5 eV/atom, force RMSE 1e-4 eV/Angstrom).
@@ -114,13 +116,66 @@ def test_edge_structures_match_oracle(ctx, launch):
             assert rmse <= 1e-4
 
 
+def test_long_rows_take_a_block_of_their_own(ctx):
+    """Three Si atoms in a triclinic cell of 1.7 A^3 and a 5 A cutoff: rows of 899-909 neighbour
+    images, whose per-centre LDS staging (FP32 adjoint with paired G5 groups: 73 KiB; FP64:
+    117 KiB) exceeds 64 KiB and runs one wave per block; next to it a two-atom cell with short
+    rows.  The G5 functions form two groups that differ only in eta (the paired walk)."""
+    A = fa.AA__BOHR
+    rc = 5.0 * A
+    rad = fa.GFunctions([2] * 2, [rc] * 2, eta=W.eta_bohr([0.05, 0.3]))
+    g5 = fa.GFunctions([5] * 4, [rc] * 4, eta=W.eta_bohr([0.02, 0.02, 0.06, 0.06]),
+                       xi=[1, 4, 1, 4], lam=[1.0, -1.0, 1.0, -1.0])
+    g = fa.GFunctions.concat([rad, g5])
+    dense = fa.Geometry(np.array([[0.1, 0.2, 0.15], [0.6, 0.5, 0.4], [0.3, 0.9, 0.8]]) * A,
+                        [14] * 3, [0] * 3,
+                        np.array([[1.15, 0.0, 0.0], [0.1, 1.2, 0.0], [0.05, 0.1, 1.25]]) * A)
+    pair = fa.Geometry(np.array([[0.0, 0.0, 0.0], [1.3, 1.1, 0.9]]) * A, [14] * 2, [0] * 2,
+                       np.array([[3.4, 0.0, 0.0], [0.3, 3.6, 0.0], [0.0, 0.2, 3.8]]) * A)
+    geos = [dense, pair]
+    dims = [len(g), 8, 1]
+    ns = fa.Netstat(g, [14], dims, 0, W.random_params(dims, 1, 17, scale=0.4),
+                    zmean=np.zeros(len(g)), zstd=np.ones(len(g)))
+    G0, _, _ = _oracle_all(geos, ns)
+    ns = fa.Netstat(g, [14], dims, 0, ns.params, zmean=np.zeros(len(g)),
+                    zstd=np.abs(G0).max(axis=0))          # features of order one
+    _, E0, F0 = _oracle_all(geos, ns)
+    ctx.upload_model(ns)
+    for prec, launch in ((fa.FNET_FP64, fa.FNET_BATCHED), (fa.FNET_FP32, fa.FNET_BATCHED),
+                         (fa.FNET_FP64, fa.FNET_PER_STRUCT)):
+        print("precision", prec, "launch", launch, flush=True)
+        b = ctx.batch(geos, prec, launch)
+        b.neighbour_list()
+        rp, _, _ = b.neighbours()
+        assert np.diff(rp)[:3].min() >= 899
+        G = b.acsf()
+        _, Ea = b.energy()
+        F = b.forces()
+        b.close()
+        if prec == fa.FNET_FP64:
+            assert np.abs(G - G0).max() <= 1e-10 * np.abs(G0).max()
+            assert np.abs(Ea - E0).max() <= 1e-10 * max(1.0, np.abs(E0).max())
+            assert np.abs(F - F0).max() <= 1e-10 * max(1.0, np.abs(F0).max())
+        else:
+            # each dense feature sums ~400 000 non-negative FP32 terms (~6 000 per lane), so
+            # the features are held to 1e-4 relative and the energies to 1e-4 of their
+            # scale; the forces to the force gate
+            grel = (np.abs(G - G0) / np.abs(G0)).max()
+            erel = np.abs(Ea - E0).max() / np.abs(E0).max()
+            rmse = np.sqrt(np.mean((F - F0) ** 2)) * HARTREE__EV / BOHR__AA
+            print("FP32 feature error", grel, "energy error", erel, "force RMSE eV/A", rmse,
+                  flush=True)
+            assert grel <= 1e-4 and erel <= 1e-4 and rmse <= 1e-4
+
+
 def test_too_dense_structure_is_refused(ctx):
-    """1 100 atoms within the cutoff of each other: rows of 1 099 entries exceed the adjoint's LDS
-    staging of one centre (64 KiB), which the host checks before the ACSF kernels run."""
+    """2 700 atoms within the cutoff of each other: rows of 2 699 entries exceed the adjoint's LDS
+    staging of one centre (173 KiB in FP32 against the 160 KiB one block may take), which the
+    host checks before the ACSF kernels run."""
     ns, rc = _model()
     ctx.upload_model(ns)
     rng = np.random.default_rng(3)
-    n = 1100
+    n = 2700
     xyz = rng.uniform(0.0, 2.5, size=(n, 3)) * fa.AA__BOHR  # box diagonal < rc
     geo = fa.Geometry(xyz, np.full(n, 1), np.zeros(n, np.int32))
     b = ctx.batch([geo], fa.FNET_FP32)
