@@ -327,9 +327,13 @@ int do_neighbours(fnet_batch* b) {
   const int fp64 = b->precision == FNET_FP64;
   HIPCHECK(ctx, launch_struct_prep(b, rc, s));
   HIPCHECK(ctx, launch_cell_bin(b, s));
-  // One pass into padded rows when the previous list gives a capacity; a row longer than it
-  // sends the batch through the fill pass below, with the counts this pass already made.
-  const bool padded = b->launch == FNET_BATCHED && b->nb_cap > 0 && ensure_padded(b);
+  // One pass into padded rows when the previous list gives a capacity (and the cell-list sort
+  // of that capacity fits a block's LDS); a row longer than it sends the batch through the fill
+  // pass below, with the counts this pass already made.
+  const bool padded = b->launch == FNET_BATCHED && b->nb_cap > 0 &&
+                      (b->ncells == 0 ||
+                       cells_lds_bytes(b->nb_cap, fp64) <= (size_t)ctx->lds_block) &&
+                      ensure_padded(b);
   if (padded) {
     HIPCHECK(ctx, launch_neigh_pad(b, rc, fp64, s));
     HIPCHECK(ctx, launch_cells_pad(b, rc, fp64, s));
@@ -370,6 +374,11 @@ int do_neighbours(fnet_batch* b) {
   }
   b->npairs = P;
   b->nl_padded = padded && b->max_nb <= b->nb_cap;
+  if (!b->nl_padded && b->ncells > 0 &&
+      cells_lds_bytes(b->max_nb, fp64) > (size_t)ctx->lds_block)
+    return fail(ctx, FNET_ERR_CAPACITY,
+                "too many neighbours around one atom for the cell-list sort (" +
+                    std::to_string(b->max_nb) + ")");
   if (b->nl_padded) {
     // the ACSF kernels read the padded rows; fnet_get_neighbours compacts them
   } else if (b->launch == FNET_PER_STRUCT) {
@@ -625,6 +634,9 @@ bool fast_step_ready(fnet_batch* b) {
   if (b->launch != FNET_BATCHED || b->N <= 0 || b->nb_cap <= 0 || b->max_nb <= 0) return false;
   if (acsf_slice_entries(b->max_nb) > b->nb_cap) return false;
   if (lds_bytes_acsf(b->max_nb, m.nZ, b->precision == FNET_FP64, 1) > (size_t)ctx->lds_block)
+    return false;
+  if (b->ncells > 0 &&
+      cells_lds_bytes(b->nb_cap, b->precision == FNET_FP64) > (size_t)ctx->lds_block)
     return false;
   if (!ensure_padded(b)) return false;
   const bool was = b->nl_padded;

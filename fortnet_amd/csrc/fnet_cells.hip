@@ -285,9 +285,7 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
   a.rc = rc;
   a.a0 = a0;
   a.a1 = a1;
-  const int need = MODE == 2 ? b->nb_cap : b->max_nb;
-  int K = 64;
-  while (K < need) K <<= 1;
+  const int K = cells_sort_capacity(MODE == 2 ? b->nb_cap : b->max_nb);
   a.K = K;
   a.cap = b->nb_cap;
   a.cnt = b->cnt;
@@ -300,7 +298,7 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
   if (WRITE) {
     const size_t per = (size_t)K * (sizeof(unsigned long long) + 3 * sizeof(int) + sizeof(V));
     while (wpb > 1 && per * wpb > 64 * 1024) wpb >>= 1;
-    if (per * wpb > 64 * 1024) return hipErrorInvalidConfiguration;
+    if (per * wpb > (size_t)b->ctx->lds_block) return hipErrorInvalidConfiguration;
     lds = per * wpb;
   }
   const unsigned grid = (unsigned)((n + wpb - 1) / wpb);
@@ -323,6 +321,17 @@ hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s) {
   k_cell_scatter<<<grid, 256, 0, s>>>(b->atom_struct, b->cellinfo, 0, b->N, b->atom_cell,
                                       b->cell_start, b->cell_count, b->cell_atoms);
   return hipGetLastError();
+}
+
+int cells_sort_capacity(int need) {
+  int K = 64;
+  while (K < need) K <<= 1;
+  return K;
+}
+
+size_t cells_lds_bytes(int need, int fp64) {
+  const size_t v = fp64 ? sizeof(double4) : sizeof(float4);
+  return (size_t)cells_sort_capacity(need) * (sizeof(unsigned long long) + 3 * sizeof(int) + v);
 }
 
 hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1,

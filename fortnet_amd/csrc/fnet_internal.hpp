@@ -351,6 +351,10 @@ constexpr int kCellMinAtoms = 64; // structures at least this large may use cell
 // with cell lists and 1.69 ms with the direct scan.
 constexpr double kCellCandidateCost = 5.0;
 hipError_t launch_cell_bin(fnet_batch* b, hipStream_t s);
+// sort capacity of one wave of the cell-list search (power of two >= need, at least 64) and its
+// LDS bytes in the writing passes
+int cells_sort_capacity(int need);
+size_t cells_lds_bytes(int need, int fp64);
 hipError_t launch_cells_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
                              hipStream_t s);

@@ -152,3 +152,42 @@ def test_triclinic_and_unwrapped(ctx):
     xyz = rng.uniform(-0.3, 1.3, size=(30, 3)) @ L  # some atoms outside the cell
     g = fa.Geometry(xyz, np.full(30, 14), np.zeros(30, np.int32), L)
     _compare(ctx, [g], 7.0)
+
+
+@pytest.mark.parametrize("precision", [fa.FNET_FP64, fa.FNET_FP32], ids=["fp64", "fp32"])
+def test_cell_list_rows_beyond_64k_of_lds(ctx, precision):
+    """A 20.5 x 15.5 x 15.5 A cell of 8 185 atoms and a 5 A cutoff (4 x 3 x 3 bins: the cell
+    lists) with 767-962 neighbours per centre.  The first list sorts each centre's hits in LDS at a
+    capacity of 1 024; the one-pass search of the next evaluation sorts at the row capacity it
+    keeps (1.25 x the longest row), 2 048 entries: 73 KiB (FP32) or 106 KiB (FP64) of LDS, beyond
+    64 KiB, in a block of its own.  Both lists are the reference's rows and give bit-identical
+    energies and forces."""
+    A = fa.AA__BOHR
+    rc = 5.0 * A
+    rng = np.random.default_rng(11)
+    lat = np.diag([20.5, 15.5, 15.5]) * A
+    n = 8185
+    xyz = rng.uniform(0.0, 1.0, size=(n, 3)) @ lat
+    geo = fa.Geometry(xyz, [14] * n, [0] * n, lat)
+    rad = fa.GFunctions([2] * 2, [rc] * 2, eta=W.eta_bohr([0.05, 0.3]))
+    g5 = fa.GFunctions([5] * 2, [rc] * 2, eta=W.eta_bohr([0.02, 0.06]), xi=[2, 2], lam=[1.0, 1.0])
+    g = fa.GFunctions.concat([rad, g5])
+    dims = [len(g), 4, 1]
+    zstd = np.array([700.0, 70.0, 4e5, 6e4])  # features of order one
+    ctx.upload_model(fa.Netstat(g, [14], dims, 0, W.random_params(dims, 1, 5, scale=0.4),
+                                zmean=np.zeros(len(g)), zstd=zstd))
+    b = ctx.batch([geo], precision)
+    b.neighbour_list()                      # counted list: sort capacity 1 024
+    rp1, j1, img1 = b.neighbours()
+    rows = np.diff(rp1)
+    print("precision", precision, "longest row", rows.max(), "mean", rows.mean(), flush=True)
+    assert 820 <= rows.max() <= 1024
+    E1, F1 = b.energy()[1], b.forces()
+    b.neighbour_list()                      # one pass into the kept rows: capacity 2 048
+    rp2, j2, img2 = b.neighbours()
+    assert np.array_equal(rp1, rp2) and np.array_equal(j1, j2) and np.array_equal(img1, img2)
+    b.update_coords(xyz, lat[None])         # the steady-state step
+    E2, F2 = b.energy()[1], b.forces()
+    assert np.array_equal(E1, E2) and np.array_equal(F1, F2)
+    _check_rows([geo], rc, rp2, j2, img2)
+    b.close()
