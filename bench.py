@@ -283,7 +283,7 @@ def main():
                     choices=["water", "si64", "si_cells", "si_bulk", "ternary"])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--train-steps", type=int, default=3)
+    ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch parameters (no autotuner)")
