@@ -730,7 +730,7 @@ __device__ __forceinline__ void pairs_adj_p(bool ipow, const Slice<T>& sl, int l
 template <typename T, int MX, int MASK, bool SAME, bool G4>
 __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int A0, int LA, int B0,
                                               int LB, T lam0, T lam1, const JkConst<T>& K,
-                                              double (&acc)[2][MX + 1]) {
+                                              T (&acc)[2][MX + 1]) {
   using V = typename Vec4<T>::type;
   PairWalk2<SAME> it;
   it.init(lane, A0, LA, B0, LB);
@@ -753,9 +753,9 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
     }
     return in ? base : (T)0;
   };
-  // G4: the slot sums go through T partial sums that are added to the FP64 ones every 16 steps
-  // (water boxes: forward 1.13 -> 1.09 ms; for the cheaper G5 pairs the extra registers cost more
-  // than the conversions saved, so they add to FP64 every step)
+  // G4: the slot sums go through partial sums that are added to the lane's sums every 16 steps
+  // (water boxes: forward 1.13 -> 1.09 ms when the lane's sums were FP64; for the cheaper G5
+  // pairs the extra registers cost more, so they add every step)
   constexpr int FLUSH = G4 ? 16 : 1;
   T part[2][MX + 1];
 #pragma unroll
@@ -767,7 +767,7 @@ __device__ __forceinline__ void pairs_fwd_lad(const Slice<T>& sl, int lane, int 
     for (int l = 0; l < 2; ++l)
 #pragma unroll
       for (int m = 0; m <= MX; ++m) {
-        if (MASK & (1 << m)) acc[l][m] += (double)part[l][m];
+        if (MASK & (1 << m)) acc[l][m] += part[l][m];
         part[l][m] = (T)0;
       }
   };
@@ -876,11 +876,13 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
   JkConst<T> K;
   K.set(rc, eta, pirc);
   if (!ADJ) {
-    double acc[2][MX + 1];
+    // each lane's slot sums in T (FP32 in the FP32 modes: a lane adds a few dozen pair terms on
+    // typical rows), the wave's sum of them in FP64
+    T acc[2][MX + 1];
 #pragma unroll
     for (int l = 0; l < 2; ++l)
 #pragma unroll
-      for (int m = 0; m <= MX; ++m) acc[l][m] = 0.0;
+      for (int m = 0; m <= MX; ++m) acc[l][m] = (T)0;
     if (same) {
       // j == k terms: cos(theta) = 1, r_jk = 0, fc(0) = 1 (acsf.F90:1420-1432)
       for (int e = lane; e < L1; e += 64) {
@@ -894,7 +896,7 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
 #pragma unroll
           for (int m = 0; m <= MX; ++m) {
             if (m > 0) p = p * p;
-            acc[l][m] += (double)(base * p);
+            acc[l][m] += base * p;
           }
         }
       }
@@ -908,7 +910,8 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
     constexpr int NA = pow2_at_least(2 * (MX + 1));
     double v[NA];
 #pragma unroll
-    for (int k = 0; k < NA; ++k) v[k] = k < 2 * (MX + 1) ? acc[k / (MX + 1)][k % (MX + 1)] : 0.0;
+    for (int k = 0; k < NA; ++k)
+      v[k] = k < 2 * (MX + 1) ? (double)acc[k / (MX + 1)][k % (MX + 1)] : 0.0;
     int k;
     const double s = wave_sum_scatter<NA>(v, lane, k);
     if ((lane & (64 / NA - 1)) == 0 && k < 2 * (MX + 1)) {
@@ -957,12 +960,12 @@ __device__ void group_lad(CGroup& grp, const Slice<T>& sl, int lane, long long i
 // Values of two G5 ladder groups over the same lists with the same cutoff and lambdas but another
 // eta, in one walk: the loads of the entries' geometry, the cosine and the ladder powers are
 // shared; each group has its own radial product (pa.y for the first group, pb.y for the partner)
-// and its own FP64 sums, which get exactly the terms, in the same order, that pairs_fwd_lad adds
+// and its own slot sums, which get exactly the terms, in the same order, that pairs_fwd_lad adds
 // for the group alone.
 template <typename T, int MX, int MASK, bool SAME>
 __device__ __forceinline__ void pairs_fwd_lad2(const Slice<T>& sl, int lane, int A0, int LA,
                                                int B0, int LB, T lam0, T lam1,
-                                               double (&acc)[2][2][MX + 1]) {
+                                               T (&acc)[2][2][MX + 1]) {
   using V = typename Vec4<T>::type;
   PairWalk2<SAME> it;
   it.init(lane, A0, LA, B0, LB);
@@ -984,16 +987,16 @@ __device__ __forceinline__ void pairs_fwd_lad2(const Slice<T>& sl, int lane, int
       const T lam = l ? lam1 : lam0;
       T p1 = (T)1 + lam * c1, p2 = (T)1 + lam * c2;
       if (MASK & 1) {
-        acc[0][l][0] += (double)(a1 * p1 + a2 * p2);
-        acc[1][l][0] += (double)(b1 * p1 + b2 * p2);
+        acc[0][l][0] += a1 * p1 + a2 * p2;
+        acc[1][l][0] += b1 * p1 + b2 * p2;
       }
 #pragma unroll
       for (int m = 1; m <= MX; ++m) {
         p1 = p1 * p1;
         p2 = p2 * p2;
         if (MASK & (1 << m)) {
-          acc[0][l][m] += (double)(a1 * p1 + a2 * p2);
-          acc[1][l][m] += (double)(b1 * p1 + b2 * p2);
+          acc[0][l][m] += a1 * p1 + a2 * p2;
+          acc[1][l][m] += b1 * p1 + b2 * p2;
         }
       }
     }
@@ -1008,13 +1011,13 @@ __device__ void group_lad_pair_fwd(CGroup& ga, CGroup& gb, const Slice<T>& sl, i
                                    int nG) {
   using V = typename Vec4<T>::type;
   const T lam0 = (T)ga.lamv[0], lam1 = (T)ga.lamv[1];
-  double acc[2][2][MX + 1];
+  T acc[2][2][MX + 1];  // each lane's slot sums in T, as in group_lad
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int l = 0; l < 2; ++l)
 #pragma unroll
-      for (int m = 0; m <= MX; ++m) acc[h][l][m] = 0.0;
+      for (int m = 0; m <= MX; ++m) acc[h][l][m] = (T)0;
   if (ga.same) {
     // j == k terms of both groups: cos(theta) = 1 (acsf.F90:1420-1432)
     for (int e = lane; e < L1; e += 64) {
@@ -1027,8 +1030,8 @@ __device__ void group_lad_pair_fwd(CGroup& ga, CGroup& gb, const Slice<T>& sl, i
 #pragma unroll
         for (int m = 0; m <= MX; ++m) {
           if (m > 0) p = p * p;
-          acc[0][l][m] += (double)(ba * p);
-          acc[1][l][m] += (double)(bb * p);
+          acc[0][l][m] += ba * p;
+          acc[1][l][m] += bb * p;
         }
       }
     }
@@ -1043,7 +1046,7 @@ __device__ void group_lad_pair_fwd(CGroup& ga, CGroup& gb, const Slice<T>& sl, i
   for (int h = 0; h < 2; ++h) {
     double v[NA];
 #pragma unroll
-    for (int k = 0; k < NA; ++k) v[k] = k < NS ? acc[h][k / (MX + 1)][k % (MX + 1)] : 0.0;
+    for (int k = 0; k < NA; ++k) v[k] = k < NS ? (double)acc[h][k / (MX + 1)][k % (MX + 1)] : 0.0;
     int k;
     const double s = wave_sum_scatter<NA>(v, lane, k);
     if ((lane & (64 / NA - 1)) == 0 && k < NS) {
@@ -1719,9 +1722,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
   acsf_wave<float, true, 1, true, true, false, PERSIST>(a);
 }
 
-// FP32 ladder forward with paired groups: bounded to 80 registers (6 waves per SIMD instead of 5),
-// which spills 6 of them, written and read back once per centre around the slot sums; on the
-// 8192 Si cells this runs faster than 5 waves without spilling
+// FP32 ladder forward with paired groups, at least 6 waves per SIMD (with FP32 slot sums per lane
+// it needs 66 registers and runs 7 without spilling)
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))
     k_acsf_lad32_fpairs(AcsfArgs a) {
   acsf_wave<float, false, 1, true, true>(a);
