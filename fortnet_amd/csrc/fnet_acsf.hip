@@ -1722,9 +1722,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
   acsf_wave<float, true, 1, true, true, false, PERSIST>(a);
 }
 
-// FP32 ladder forward with paired groups, at least 6 waves per SIMD (with FP32 slot sums per lane
-// it needs 66 registers and runs 7 without spilling)
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))
+// FP32 ladder forward with paired groups, bounded to 64 registers (8 waves per SIMD), which its
+// FP32 slot sums per lane meet without spilling (66 unbounded, 7 waves)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8)))
     k_acsf_lad32_fpairs(AcsfArgs a) {
   acsf_wave<float, false, 1, true, true>(a);
 }

@@ -670,15 +670,17 @@ int enqueue_fast_step(fnet_batch* b, bool events) {
   if (events) ev_record(ctx, 0);
   HIPCHECK(ctx, launch_struct_prep(b, rc, s));
   HIPCHECK(ctx, launch_cell_bin(b, s));
-  HIPCHECK(ctx, launch_neigh_pad(b, rc, fp64, s));
-  HIPCHECK(ctx, launch_cells_pad(b, rc, fp64, s));
+  // the search clears the mirror slot of every entry it writes, so the mirror search needs no fill
+  // of the whole padded array first
+  HIPCHECK(ctx, launch_neigh_pad(b, rc, fp64, s, b->mir));
+  HIPCHECK(ctx, launch_cells_pad(b, rc, fp64, s, b->mir));
   HIPCHECK(ctx, launch_row_max(b->cnt, b->N, b->d_maxnb, s));
   // The mirror entries need only the list: their search (latency-bound) runs on the side stream
   // beside the ACSF forward and the networks (VALU- and matrix-bound), and the adjoint waits for
   // it.  Under graph capture the event pair becomes a fork and a join of the graph.
   HIPCHECK(ctx, hipEventRecord(ctx->fork_ev, s));
   HIPCHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-  HIPCHECK(ctx, launch_mirror(b, 0, b->N, layout_entries(b), ctx->side));
+  HIPCHECK(ctx, launch_mirror(b, 0, b->N, 0, ctx->side));
   HIPCHECK(ctx, hipEventRecord(ctx->join_ev, ctx->side));
   if (events) ev_record(ctx, 1);
   HIPCHECK(ctx, launch_acsf_fwd(b, m, 0, b->N, fp64, s));

@@ -84,6 +84,7 @@ struct CellArgs {
   int* nj;
   int* nimg;
   void* nd;
+  int* mir;  // padded rows: each entry's mirror slot set to -1 when given
 };
 
 // MODE: 0 writes the row lengths; 1 writes the sorted rows at their CSR offsets; 2 writes row i
@@ -263,11 +264,13 @@ __global__ void __launch_bounds__(256) k_neigh_cells(CellArgs a) {
     a.nj[rb + e] = pj[q];
     a.nimg[rb + e] = (int)key[e];  // the entry's key (image + 1) n + (j - i) mod n
     out[rb + e] = pnd[q];
+    if (MODE == 2 && a.mir) a.mir[rb + e] = -1;
   }
 }
 
 template <int MODE, typename V>
-hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s) {
+hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s,
+                          int* mir = nullptr) {
   constexpr bool WRITE = MODE != 0;
   const long long n = a1 - a0;
   if (n <= 0) return hipSuccess;
@@ -293,6 +296,7 @@ hipError_t launch_cells_t(fnet_batch* b, double rc, long long a0, long long a1, 
   a.nj = MODE == 2 ? b->pnj : b->nj;
   a.nimg = MODE == 2 ? b->pnimg : b->nimg;
   a.nd = MODE == 2 ? b->pnd : b->nd;
+  a.mir = MODE == 2 ? mir : nullptr;
   size_t lds = 0;
   int wpb = 4;
   if (WRITE) {
@@ -347,10 +351,10 @@ hipError_t launch_cells_fill(fnet_batch* b, double rc, long long a0, long long a
               : launch_cells_t<1, float4>(b, rc, a0, a1, s);
 }
 
-hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s) {
+hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s, int* mir) {
   if (b->ncells == 0) return hipSuccess;
-  return fp64 ? launch_cells_t<2, double4>(b, rc, 0, b->N, s)
-              : launch_cells_t<2, float4>(b, rc, 0, b->N, s);
+  return fp64 ? launch_cells_t<2, double4>(b, rc, 0, b->N, s, mir)
+              : launch_cells_t<2, float4>(b, rc, 0, b->N, s, mir);
 }
 
 }  // namespace fnet

@@ -194,7 +194,8 @@ hipError_t launch_zero_int(int* p, int n, hipStream_t s) {
 }
 
 // Mirror entries of atoms [a0, a1); `entries` > 0 first fills the whole mirror array with -1
-// (a range of atoms whose rows are alone in their structures may skip it after the first).
+// (a range of atoms whose rows are alone in their structures may skip it after the first, and the
+// steady-state step skips it: its padded search sets every written entry's slot to -1).
 hipError_t launch_mirror(fnet_batch* b, long long a0, long long a1, long long entries,
                          hipStream_t s) {
   if (entries > 0)

@@ -334,8 +334,11 @@ hipError_t launch_struct_prep(fnet_batch* b, double rc, hipStream_t s);
 hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long a1, hipStream_t s);
 hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a1, int fp64,
                              hipStream_t s);
-hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s);
-hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s);
+// padded rows; a non-null mir (sized for the padded layout) gets -1 at every entry written
+hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s,
+                            int* mir = nullptr);
+hipError_t launch_cells_pad(fnet_batch* b, double rc, int fp64, hipStream_t s,
+                            int* mir = nullptr);
 hipError_t launch_compact_rows(fnet_batch* b, int fp64, hipStream_t s);
 hipError_t launch_scan(int* cnt, int* rowptr, long long n, int* tmp, hipStream_t s,
                        int* maxout);

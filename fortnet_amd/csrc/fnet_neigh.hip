@@ -176,8 +176,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
             const double* __restrict__ lat, double rc, long long a0, long long a1,
             int* __restrict__ cnt, const int* __restrict__ rowptr,
             int* __restrict__ nj, int* __restrict__ nimg, V* __restrict__ nd,
-            const int4* __restrict__ cellinfo, int cap) {
-  // nimg receives each entry's key (image + 1) n + (j - i) mod n
+            const int4* __restrict__ cellinfo, int cap, int* __restrict__ mir) {
+  // nimg receives each entry's key (image + 1) n + (j - i) mod n; padded rows also set each
+  // entry's mirror slot to -1 (no mirror) when mir is given, for the mirror search to fill in
 #pragma clang fp contract(off)
   constexpr bool WRITE = MODE != kCount;
   const int lane = threadIdx.x & 63;
@@ -209,6 +210,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
       v.z = nd2;
       entry_d2(v, d2);
       nd[pos] = v;
+      if (MODE == kPadded && mir) mir[pos] = -1;
     }
     count += __popcll(mask);
   };
@@ -460,7 +462,7 @@ hipError_t launch_neigh_count(fnet_batch* b, double rc, long long a0, long long 
   const unsigned grid = (unsigned)((n + 3) / 4);
   k_neigh<kCount, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
                                                  b->lat, rc, a0, a1, b->cnt, nullptr, nullptr,
-                                                 nullptr, nullptr, b->cellinfo, 0);
+                                                 nullptr, nullptr, b->cellinfo, 0, nullptr);
   return hipGetLastError();
 }
 
@@ -472,16 +474,18 @@ hipError_t launch_neigh_fill(fnet_batch* b, double rc, long long a0, long long a
   if (fp64) {
     k_neigh<kFill, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
                                                  b->lat, rc, a0, a1, nullptr, b->rowptr, b->nj,
-                                                 b->nimg, (double4*)b->nd, b->cellinfo, 0);
+                                                 b->nimg, (double4*)b->nd, b->cellinfo, 0,
+                                                 nullptr);
   } else {
     k_neigh<kFill, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
                                                 b->lat, rc, a0, a1, nullptr, b->rowptr, b->nj,
-                                                b->nimg, (float4*)b->nd, b->cellinfo, 0);
+                                                b->nimg, (float4*)b->nd, b->cellinfo, 0,
+                                                nullptr);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s) {
+hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s, int* mir) {
   const long long n = b->N;
   if (n <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((n + 3) / 4);
@@ -489,12 +493,12 @@ hipError_t launch_neigh_pad(fnet_batch* b, double rc, int fp64, hipStream_t s) {
     k_neigh<kPadded, double4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
                                                    b->lat, rc, 0, n, b->cnt, nullptr, b->pnj,
                                                    b->pnimg, (double4*)b->pnd, b->cellinfo,
-                                                   b->nb_cap);
+                                                   b->nb_cap, mir);
   } else {
     k_neigh<kPadded, float4><<<grid, 256, 0, s>>>(b->xyz, b->off, b->atom_struct, b->sinfo,
                                                   b->lat, rc, 0, n, b->cnt, nullptr, b->pnj,
                                                   b->pnimg, (float4*)b->pnd, b->cellinfo,
-                                                  b->nb_cap);
+                                                  b->nb_cap, mir);
   }
   return hipGetLastError();
 }
