@@ -1699,8 +1699,8 @@ __global__ void __launch_bounds__(512) k_acsf(AcsfArgs a) {
   acsf_wave<T, ADJ, MQ, LAD, PAIRS>(a);
 }
 
-// The FP32 exponent-ladder kernels fit in 96 registers without spilling, which leaves room for 5
-// waves per SIMD instead of 4 (the compiler's choice without the bound).
+// The FP32 exponent-ladder kernels, at least 5 waves per SIMD (the bound once kept them from the
+// compiler's 4; with the forward's lane sums in FP32 they need 68-69 registers and run 7).
 template <bool ADJ>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5)))
     k_acsf_lad32(AcsfArgs a) {
